@@ -1,0 +1,112 @@
+/*
+ * tdtm -- MI355X-native 2D template matching (2DTM) search engine, C ABI.
+ *
+ * Plain pointers and sizes only.  All device pointers are HIP device allocations
+ * (e.g. torch tensors' data_ptr()) on the current device; `stream` is a hipStream_t
+ * (NULL = default stream).  Every entry point returns 0 on success or a non-zero
+ * code; tdtm_last_error() then returns a thread-local message.  Work is enqueued
+ * asynchronously on `stream`.
+ *
+ * Layouts (row-major, C order):
+ *   image_dft     complex64 (H, W/2+1)       rfft of the pre-processed micrograph, DC = 0
+ *   template_dft  complex64 (h, h, h/2+1)    3D rfft of the template, fftshifted in dims 0,1
+ *   ctf_filters   float32   (C, D, h, h/2+1)
+ *   whitening     float32   (h, h/2+1)
+ *   euler_deg     float32   (O, 3)           (phi, theta, psi), ZYZ intrinsic, degrees
+ *   statistics    (H, W) each: mip f32, best_index i32, sum_hi/sum_lo f32 (compensated
+ *                 pair), sq_hi/sq_lo f32 (compensated pair)
+ */
+#ifndef TDTM_H
+#define TDTM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tdtm_match_plan* tdtm_match_plan_t;
+
+/* Version of the ABI (bumped on any signature change). */
+int tdtm_abi_version(void);
+
+/* Thread-local message for the last failing call on this thread. */
+const char* tdtm_last_error(void);
+
+/*
+ * Plan for a whole-orientation search: buffers and twiddle tables for an H x W image,
+ * an h^3 template and up to max_orientation_batch orientations x num_defocus x
+ * num_pixel correlations per step.  Replaces the per-device set-up inside
+ * _core_match_template_single_gpu (src/leopard_em/backend/core_match_template.py:369-486).
+ * Supported sizes: H, W in {64 .. 4096} powers of two; h in {32 .. 512} powers of two,
+ * h <= min(H, W).
+ */
+int tdtm_match_plan_create(int image_h, int image_w, int template_h, int max_orientation_batch,
+                           int num_defocus, int num_pixel, tdtm_match_plan_t* out_plan);
+int tdtm_match_plan_destroy(tdtm_match_plan_t plan);
+/* Device bytes held by the plan. */
+int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
+
+/*
+ * Load the pre-processed image spectrum (re-tiled and scaled by 1/(H W) inside the
+ * plan).  Replaces image_dft.to(device) at core_match_template.py:448.
+ */
+int tdtm_match_set_image(tdtm_match_plan_t plan, const void* image_dft, void* stream);
+
+/*
+ * projective_filters = ctf_filters * whitening (core_match_template.py:274), kept in
+ * the plan.
+ */
+int tdtm_match_set_filters(tdtm_match_plan_t plan, const float* ctf_filters, const float* whitening,
+                           void* stream);
+
+/*
+ * mip = -inf, best_index = -1, sums = 0 over n pixels
+ * (core_match_template.py:472-486).
+ */
+int tdtm_stats_init(int64_t n, float* mip, int32_t* best_index, float* sum_hi, float* sum_lo,
+                    float* sq_hi, float* sq_lo, void* stream);
+
+/*
+ * One orientation batch [start, start+count) of the search: Fourier slice extraction,
+ * projective filter, small irfft2, projection normalisation, padded FFT
+ * cross-correlation against the image and the MIP / argmax / sum / sum-of-squares
+ * update, fused so correlation maps never reach HBM.  Replaces one iteration of the
+ * loop at core_match_template.py:504-550, i.e.
+ * do_streamed_orientation_cross_correlate (cross_correlation.py:13-154) followed by
+ * do_iteration_statistics_updates (utils.py:191-256).  Global search index of
+ * correlation (c, d, o) is c*D*O + d*O + o with O = num_orientations.
+ */
+int tdtm_match_step(tdtm_match_plan_t plan, const void* template_dft, const float* euler_deg,
+                    int num_orientations, int start, int count, float* mip, int32_t* best_index,
+                    float* sum_hi, float* sum_lo, float* sq_hi, float* sq_lo, void* stream);
+
+/*
+ * Final statistics (FP64): mean = sum/N, std = sqrt(max(sumsq/N - mean^2, 0)),
+ * scaled_mip = (mip - mean)/std (0 where std == 0), and best index decoded to
+ * (phi, theta, psi, defocus).  Replaces decode_global_search_index and scale_mip
+ * (process_results.py:67-183).  Output maps are float32 (n pixels).
+ */
+int tdtm_match_finalize(int64_t n, int64_t total_positions, const float* mip, const int32_t* best_index,
+                        const float* sum_hi, const float* sum_lo, const float* sq_hi, const float* sq_lo,
+                        const float* euler_deg, int num_orientations, const float* defocus_values,
+                        int num_defocus, float* out_scaled_mip, float* out_mean, float* out_std,
+                        float* out_phi, float* out_theta, float* out_psi, float* out_defocus,
+                        void* stream);
+
+/*
+ * Cross-device merge of partial statistics (packed max with first-operand precedence
+ * on ties, compensated sums): a <- merge(a, b).  Replaces
+ * aggregate_distributed_results (process_results.py:7-64) for device-resident maps.
+ */
+int tdtm_stats_merge(int64_t n, float* mip_a, int32_t* idx_a, float* sum_hi_a, float* sum_lo_a,
+                     float* sq_hi_a, float* sq_lo_a, const float* mip_b, const int32_t* idx_b,
+                     const float* sum_hi_b, const float* sum_lo_b, const float* sq_hi_b,
+                     const float* sq_lo_b, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TDTM_H */

@@ -157,20 +157,19 @@ __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
 template <int N>
 struct LdsSize {
-  static constexpr int floats = N + N / 16;  // per component
+  static constexpr int floats = N + N / 16 + 1;  // per component (index N is valid)
 };
 
 // out[m] = sum_{j < NZJ} a[j] W_R^{jm}: first pass when only the leading NZJ inputs of
 // every butterfly can be non-zero (zero-padded transforms)
 template <int R, int NZJ, bool INV>
 __device__ __forceinline__ void dft_sparse(float2 (&a)[R], const float2* __restrict__ tw) {
-  if constexpr (NZJ >= R) {
+  if constexpr (NZJ >= R || NZJ > 2) {
     dft<R, INV>(a);
   } else if constexpr (NZJ == 1) {
 #pragma unroll
     for (int m = 1; m < R; ++m) a[m] = a[0];
   } else {
-    static_assert(NZJ == 2, "sparse first pass supports 1 or 2 leading inputs");
     const float2 a0 = a[0], a1 = a[1];
     constexpr int step = TW_N / R;
 #pragma unroll

@@ -1,0 +1,793 @@
+// Whole-orientation 2DTM search on gfx950: kernels and C ABI.
+//
+// One search step (B orientations x D defocus x C pixel sizes = K correlations):
+//   k_slice        trilinear Fourier central slice, Hermitian fold, DC zero, contrast
+//                  flip and the projection's quadrant swap folded in as (-1)^(ky+kx)
+//   k_proj_cols    filter multiply + inverse FFT along y of every slice column
+//   k_proj_rows    c2r along x -> real projection rows + normalisation partial sums
+//   k_proj_norm    per-projection edge mean / scale (affine normalisation)
+//   k_rowfwd       normalised projection rows -> zero-padded r2c along x (W points)
+//   k_colpass      per column group: pruned forward FFT along y, multiply by the image
+//                  spectrum conj, inverse FFT along y -> tiled half-spectrum intermediate
+//   k_rowpass      per row group: c2r along x fused with MIP / argmax / sum / sum-sq
+//                  update; the per-correlation correlation maps never reach HBM
+// The intermediate between k_colpass and k_rowpass (one complex half spectrum per
+// correlation) is the only full-size array written and read per correlation.  It is
+// stored in 4-row x 2-column tiles (64 bytes) so that the column pass writes, and
+// the row pass reads, whole 64-byte segments.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fft.hpp"
+#include "tdtm.h"
+
+namespace tdtm {
+
+constexpr int V = 8;   // values per thread in every FFT
+constexpr int CW = 2;  // columns per column-pass block (tile width)
+constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
+
+thread_local std::string g_err;
+
+static int fail(const std::string& msg) {
+  g_err = msg;
+  return 1;
+}
+
+#define TDTM_HIP(call)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TDTM_LAUNCHED()                                                                 \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess) return fail(std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// ------------------------------------------------------------------------- kernels
+__global__ void k_tile_image(const float2* __restrict__ X, int H, int Wr, int G, float scale,
+                             float2* __restrict__ Xt) {
+  const long total = (long)G * H * CW;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int c = e % CW;
+  const int ky = (e / CW) % H;
+  const int g = e / ((long)CW * H);
+  const int kx = g * CW + c;
+  float2 v = make_float2(0.f, 0.f);
+  if (kx < Wr) v = c_scale(X[(long)ky * Wr + kx], scale);
+  Xt[e] = v;
+}
+
+__global__ void k_filters(const float* __restrict__ ctf, const float* __restrict__ wf, long plane, long total,
+                          float* __restrict__ out) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  out[e] = ctf[e] * wf[e % plane];
+}
+
+// Trilinear central slice of the fftshifted 3D rfft; output unshifted in y.
+__global__ void k_slice(const float2* __restrict__ vol, int n, const float* __restrict__ euler, int o0, int nb,
+                        float2* __restrict__ S) {
+  const int wr = n / 2 + 1;
+  const long total = (long)nb * n * wr;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int kx = idx % wr;
+  const int r = (idx / wr) % n;
+  const int b = idx / ((long)wr * n);
+  const float d2r = 0.017453292519943295f;
+  float sph, cph, sth, cth, sps, cps;
+  sincosf(euler[3 * (o0 + b) + 0] * d2r, &sph, &cph);
+  sincosf(euler[3 * (o0 + b) + 1] * d2r, &sth, &cth);
+  sincosf(euler[3 * (o0 + b) + 2] * d2r, &sps, &cps);
+  // columns 0 and 1 of R = Rz(phi) Ry(theta) Rz(psi)
+  const float r00 = cph * cth * cps - sph * sps, r01 = -cph * cth * sps - sph * cps;
+  const float r10 = sph * cth * cps + cph * sps, r11 = -sph * cth * sps + cph * cps;
+  const float r20 = -sth * cps, r21 = sth * sps;
+  const float fx = (float)kx / (float)n;
+  const float fy = (float)(r < n / 2 ? r : r - n) / (float)n;
+  float rx = r00 * fx + r01 * fy, ry = r10 * fx + r11 * fy, rz = r20 * fx + r21 * fy;
+  const bool flip = rx < 0.f;
+  if (flip) {
+    rx = -rx;
+    ry = -ry;
+    rz = -rz;
+  }
+  const float cz = rz * n + (float)(n / 2), cy = ry * n + (float)(n / 2), cx = rx * n;
+  float2 val = make_float2(0.f, 0.f);
+  if (cz >= 0.f && cz <= (float)(n - 1) && cy >= 0.f && cy <= (float)(n - 1) && cx >= 0.f &&
+      cx <= (float)(wr - 1)) {
+    const int z0 = (int)floorf(cz), y0 = (int)floorf(cy), x0 = (int)floorf(cx);
+    const float fz = cz - z0, fyy = cy - y0, fxx = cx - x0;
+    const int z1 = min(z0 + 1, n - 1), y1 = min(y0 + 1, n - 1), x1 = min(x0 + 1, wr - 1);
+    const long sz = (long)n * wr;
+    const float2 v000 = vol[z0 * sz + (long)y0 * wr + x0], v001 = vol[z0 * sz + (long)y0 * wr + x1];
+    const float2 v010 = vol[z0 * sz + (long)y1 * wr + x0], v011 = vol[z0 * sz + (long)y1 * wr + x1];
+    const float2 v100 = vol[z1 * sz + (long)y0 * wr + x0], v101 = vol[z1 * sz + (long)y0 * wr + x1];
+    const float2 v110 = vol[z1 * sz + (long)y1 * wr + x0], v111 = vol[z1 * sz + (long)y1 * wr + x1];
+    const float w000 = (1 - fz) * (1 - fyy) * (1 - fxx), w001 = (1 - fz) * (1 - fyy) * fxx;
+    const float w010 = (1 - fz) * fyy * (1 - fxx), w011 = (1 - fz) * fyy * fxx;
+    const float w100 = fz * (1 - fyy) * (1 - fxx), w101 = fz * (1 - fyy) * fxx;
+    const float w110 = fz * fyy * (1 - fxx), w111 = fz * fyy * fxx;
+    val.x = w000 * v000.x + w001 * v001.x + w010 * v010.x + w011 * v011.x + w100 * v100.x + w101 * v101.x +
+            w110 * v110.x + w111 * v111.x;
+    val.y = w000 * v000.y + w001 * v001.y + w010 * v010.y + w011 * v011.y + w100 * v100.y + w101 * v101.y +
+            w110 * v110.y + w111 * v111.y;
+  }
+  if (flip) val.y = -val.y;
+  if (r == 0 && kx == 0) val = make_float2(0.f, 0.f);
+  // contrast flip (-1) times (-1)^(ky+kx): the spatial quadrant swap of the projection
+  const float sgn = ((r + kx) & 1) ? 1.f : -1.f;
+  S[idx] = c_scale(val, sgn);
+}
+
+// Inverse FFT along y of (slice * filter) for CP columns per block.
+template <int HS, int CP>
+__global__ __launch_bounds__(CP* HS / V) void k_proj_cols(const float2* __restrict__ S,
+                                                          const float* __restrict__ filt, int nb, int nd,
+                                                          const float2* __restrict__ tw, float2* __restrict__ Tb) {
+  constexpr int T = HS / V;
+  constexpr int WR = HS / 2 + 1;
+  constexpr int LF = LdsSize<HS>::floats;
+  __shared__ float lds[CP * 2 * LF];
+  const int c = threadIdx.x % CP, t = threadIdx.x / CP;
+  const int kx = blockIdx.x * CP + c;
+  const int k = blockIdx.y;
+  const int b = k % nb, d = (k / nb) % nd, ci = k / (nb * nd);
+  const bool ok = kx < WR;
+  const float2* s = S + (long)b * HS * WR;
+  const float* f = filt + ((long)ci * nd + d) * HS * WR;
+  float2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int ky = t + i * T;
+    v[i] = ok ? c_scale(s[(long)ky * WR + kx], f[(long)ky * WR + kx]) : make_float2(0.f, 0.f);
+  }
+  fft_wg<HS, V, true>(v, lds + c * 2 * LF, lds + c * 2 * LF + LF, t, tw);
+  if (ok) {
+    float2* o = Tb + (long)k * HS * WR;
+#pragma unroll
+    for (int i = 0; i < V; ++i) o[(long)(t + i * T) * WR + kx] = v[i];
+  }
+}
+
+// c2r along x of each row (imaginary parts of DC / Nyquist ignored, as pocketfft's
+// c2r does) + per-row partial sums for the projection normalisation.
+template <int HS, int RPB>
+__global__ __launch_bounds__(RPB*(HS / 2) / V) void k_proj_rows(const float2* __restrict__ Tb,
+                                                                const float2* __restrict__ tw,
+                                                                float* __restrict__ P, float* __restrict__ part) {
+  constexpr int M = HS / 2;
+  constexpr int T = M / V;
+  constexpr int WR = M + 1;
+  constexpr int LF = LdsSize<M>::floats;
+  __shared__ float lds[RPB * 2 * LF];
+  __shared__ float2 stg[RPB * WR];
+  const int rl = threadIdx.x / T, t = threadIdx.x % T;
+  const int y = blockIdx.x * RPB + rl;
+  const int k = blockIdx.y;
+  const float2* row = Tb + ((long)k * HS + y) * WR;
+  for (int e = t; e < WR; e += T) stg[rl * WR + e] = row[e];
+  __syncthreads();
+  float2 v[V];
+  const float2* g = stg + rl * WR;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int m = t + i * T;
+    float2 A = g[m];
+    float2 Bc = g[M - m];
+    if (m == 0) {
+      A.y = 0.f;
+      Bc.y = 0.f;
+    }
+    Bc = c_conj(Bc);
+    const float2 w = c_conj(tw[m * (TW_N / HS)]);  // e^{+2 pi i m / HS}
+    const float2 dsum = c_add(A, Bc), ddif = c_sub(A, Bc);
+    const float2 od = c_mul(w, ddif);
+    v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);  // dsum + i*od
+  }
+  __syncthreads();
+  fft_wg<M, V, true>(v, lds + rl * 2 * LF, lds + rl * 2 * LF + LF, t, tw);
+  const float scale = 1.0f / ((float)HS * (float)HS);
+  float s1 = 0.f, s2 = 0.f;
+  float* prow = P + ((long)k * HS + y) * HS;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int nn = t + i * T;
+    const float2 p = c_scale(v[i], scale);
+    reinterpret_cast<float2*>(prow)[nn] = p;
+    s1 += p.x + p.y;
+    s2 += p.x * p.x + p.y * p.y;
+  }
+  float se = 0.f;
+  if (t == 0) se += v[0].x * scale;            // p[0]
+  if (t == T - 1) se += v[V - 1].y * scale;    // p[HS-1]
+#pragma unroll
+  for (int off = T / 2; off > 0; off >>= 1) {
+    s1 += __shfl_xor(s1, off, T);
+    s2 += __shfl_xor(s2, off, T);
+    se += __shfl_xor(se, off, T);
+  }
+  if (t == 0) {
+    float* pp = part + ((long)k * HS + y) * 3;
+    pp[0] = s1;
+    pp[1] = s2;
+    pp[2] = (y == 0 || y == HS - 1) ? s1 : se;
+  }
+}
+
+// Edge mean e and scale a so that a*(p - e) is the reference's normalised projection
+// (backend/utils.py:154-187, incl. the doubled relative-size factor on the mean).
+__global__ void k_proj_norm(const float* __restrict__ part, int hs, int H, int W, float2* __restrict__ ae) {
+  __shared__ double red[3][256];
+  const int k = blockIdx.x;
+  double s1 = 0, s2 = 0, se = 0;
+  for (int y = threadIdx.x; y < hs; y += blockDim.x) {
+    const float* pp = part + ((long)k * hs + y) * 3;
+    s1 += pp[0];
+    s2 += pp[1];
+    se += pp[2];
+  }
+  red[0][threadIdx.x] = s1;
+  red[1][threadIdx.x] = s2;
+  red[2][threadIdx.x] = se;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + off];
+      red[1][threadIdx.x] += red[1][threadIdx.x + off];
+      red[2][threadIdx.x] += red[2][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double n = (double)hs * hs;
+    const double e = red[2][0] / (4.0 * hs - 4.0);
+    const double rel = n / ((double)H * W);
+    const double mu = (red[0][0] / n - e) * rel * rel;
+    const double cc = e + mu;
+    double var = red[1][0] - 2.0 * cc * red[0][0] + n * cc * cc;
+    var += ((double)H - hs) * ((double)W - hs) * mu * mu;
+    const double a = sqrt((double)H * W) / sqrt(var > 1e-8 ? var : 1e-8);
+    ae[k] = make_float2((float)a, (float)e);
+  }
+}
+
+// Zero-padded r2c along x (WS points, HS non-zero) of the normalised projection rows.
+template <int WS, int NZJ>
+__global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict__ P, const float2* __restrict__ ae,
+                                                        int hs, int G, const float2* __restrict__ tw,
+                                                        float2* __restrict__ Rb) {
+  constexpr int M = WS / 2;
+  constexpr int T = M / V;
+  constexpr int LF = LdsSize<M>::floats;
+  __shared__ float lds[2 * LF];
+  float* lre = lds;
+  float* lim = lds + LF;
+  const int y = blockIdx.x;
+  const int k = blockIdx.y;
+  const int t = threadIdx.x;
+  const float2 s = ae[k];
+  const float2* prow = reinterpret_cast<const float2*>(P + ((long)k * hs + y) * hs);
+  float2 v[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int nn = t + i * T;
+    float2 z = make_float2(0.f, 0.f);
+    if (i < NZJ && nn < hs / 2) {
+      const float2 p = prow[nn];
+      z = make_float2(s.x * (p.x - s.y), s.x * (p.y - s.y));
+    }
+    v[i] = z;
+  }
+  fft_wg<M, V, false, NZJ>(v, lre, lim, t, tw);
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const int p = lds_pad(t + i * T);
+    lre[p] = v[i].x;
+    lim[p] = v[i].y;
+  }
+  __syncthreads();
+  float2* out = Rb + (long)k * G * hs * CW;
+  for (int kx = t; kx < G * CW; kx += T) {
+    float2 X = make_float2(0.f, 0.f);
+    if (kx <= M) {
+      const int a = kx & (M - 1), bi = (M - kx) & (M - 1);
+      const float2 A = make_float2(lre[lds_pad(a)], lim[lds_pad(a)]);
+      const float2 Bc = make_float2(lre[lds_pad(bi)], -lim[lds_pad(bi)]);
+      const float2 E = c_scale(c_add(A, Bc), 0.5f);
+      const float2 O = c_scale(c_sub(A, Bc), 0.5f);
+      const float2 wo = c_mul(tw[kx * (TW_N / WS)], O);  // e^{-2 pi i kx/WS} * O
+      X = make_float2(E.x + wo.y, E.y - wo.x);           // E - i*wo
+    }
+    out[((long)(kx / CW) * hs + y) * CW + (kx % CW)] = X;
+  }
+}
+
+// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
+// conjugate against the image spectrum column, inverse FFT along y; tiled store.
+template <int HH, int NZJ>
+__global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
+                                                        int hs, int G, int K, int kchunk,
+                                                        const float2* __restrict__ tw, float2* __restrict__ Gb) {
+  constexpr int T = HH / V;
+  constexpr int LF = LdsSize<HH>::floats;
+  __shared__ float lds[CW * 2 * LF];
+  const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+  float* lre = lds + c * 2 * LF;
+  float* lim = lre + LF;
+  const int g = blockIdx.x;
+  const int k0 = blockIdx.y * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  float2 xi[V];
+#pragma unroll
+  for (int i = 0; i < V; ++i) xi[i] = Xt[((long)g * HH + t + i * T) * CW + c];
+  const bool dc = (g == 0 && c == 0 && t == 0);
+  for (int k = k0; k < k1; ++k) {
+    const float2* rin = Rb + ((long)k * G + g) * hs * CW;
+    float2 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int yy = t + i * T;
+      v[i] = (i < NZJ && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+    }
+    fft_wg<HH, V, false, NZJ>(v, lre, lim, t, tw);
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[i] = c_mulc(xi[i], v[i]);
+    if (dc) v[0] = make_float2(0.f, 0.f);
+    fft_wg<HH, V, true>(v, lre, lim, t, tw);
+    float2* gout = Gb + (long)k * HH * G * CW;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int yy = t + i * T;
+      gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+    }
+  }
+}
+
+__device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
+  const float s = hi + b;
+  const float bb = s - hi;
+  const float err = (hi - (s - bb)) + (b - bb);
+  hi = s;
+  lo += err;
+}
+
+// Row pass: c2r along x (WS points) for TR rows, fused statistics update over all K
+// correlations of the step, then one merge into the running maps.
+template <int WS>
+__global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
+                                                            int nb, int nd, int O, int o0,
+                                                            const float2* __restrict__ tw, float* __restrict__ mip,
+                                                            int* __restrict__ bidx, float* __restrict__ shi,
+                                                            float* __restrict__ slo, float* __restrict__ qhi,
+                                                            float* __restrict__ qlo) {
+  constexpr int M = WS / 2;
+  constexpr int T = M / V;
+  constexpr int LF = LdsSize<M>::floats;
+  __shared__ float lds[TR * 2 * LF];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  float* lre = lds + r * 2 * LF;
+  float* lim = lre + LF;
+  const int p = blockIdx.x;  // row-tile index
+  const int y = p * TR + r;
+  float m[2 * V], s[2 * V], q[2 * V];
+  int ix[2 * V];
+#pragma unroll
+  for (int j = 0; j < 2 * V; ++j) {
+    m[j] = -INFINITY;
+    s[j] = 0.f;
+    q[j] = 0.f;
+    ix[j] = -1;
+  }
+  const int tile_elems = G * TR * CW;
+  for (int k = 0; k < K; ++k) {
+    const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
+    for (int e = threadIdx.x; e < tile_elems; e += blockDim.x) {
+      const int cc = e % CW, rr = (e / CW) % TR, gg = e / (CW * TR);
+      const int kx = gg * CW + cc;
+      if (kx <= M) {
+        const float2 val = gin[e];
+        const int pp = lds_pad(kx);
+        lds[rr * 2 * LF + pp] = val.x;
+        lds[rr * 2 * LF + LF + pp] = val.y;
+      }
+    }
+    __syncthreads();
+    float2 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int mm = t + i * T;
+      float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
+      const int bm = M - mm;
+      float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
+      if (mm == 0) {
+        A.y = 0.f;
+        Bc.y = 0.f;
+      }
+      const float2 w = c_conj(tw[mm * (TW_N / WS)]);
+      const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
+      v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
+    }
+    __syncthreads();
+    fft_wg<M, V, true>(v, lre, lim, t, tw);
+    const int bb = k % nb, dd = (k / nb) % nd, ci = k / (nb * nd);
+    const int gidx = ci * nd * O + dd * O + o0 + bb;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const float a0 = v[i].x, a1 = v[i].y;
+      if (a0 > m[2 * i]) {
+        m[2 * i] = a0;
+        ix[2 * i] = gidx;
+      }
+      if (a1 > m[2 * i + 1]) {
+        m[2 * i + 1] = a1;
+        ix[2 * i + 1] = gidx;
+      }
+      s[2 * i] += a0;
+      s[2 * i + 1] += a1;
+      q[2 * i] = fmaf(a0, a0, q[2 * i]);
+      q[2 * i + 1] = fmaf(a1, a1, q[2 * i + 1]);
+    }
+  }
+  const long rowbase = (long)y * WS;
+#pragma unroll
+  for (int i = 0; i < V; ++i) {
+    const long px = rowbase + 2 * (t + i * T);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const long e = px + h;
+      const int j = 2 * i + h;
+      const float mo = mip[e];
+      if (m[j] > mo) {
+        mip[e] = m[j];
+        bidx[e] = ix[j];
+      }
+      float sh = shi[e], sl = slo[e];
+      two_sum(sh, sl, s[j]);
+      shi[e] = sh;
+      slo[e] = sl;
+      float qh = qhi[e], ql = qlo[e];
+      two_sum(qh, ql, q[j]);
+      qhi[e] = qh;
+      qlo[e] = ql;
+    }
+  }
+}
+
+__global__ void k_stats_init(long n, float* mip, int* idx, float* a, float* b, float* c, float* d) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  mip[e] = -INFINITY;
+  idx[e] = -1;
+  a[e] = 0.f;
+  b[e] = 0.f;
+  c[e] = 0.f;
+  d[e] = 0.f;
+}
+
+__global__ void k_stats_merge(long n, float* mip, int* idx, float* shi, float* slo, float* qhi, float* qlo,
+                              const float* mip_b, const int* idx_b, const float* shi_b, const float* slo_b,
+                              const float* qhi_b, const float* qlo_b) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  if (mip_b[e] > mip[e]) {
+    mip[e] = mip_b[e];
+    idx[e] = idx_b[e];
+  }
+  float h = shi[e], l = slo[e];
+  two_sum(h, l, shi_b[e]);
+  l += slo_b[e];
+  shi[e] = h;
+  slo[e] = l;
+  h = qhi[e];
+  l = qlo[e];
+  two_sum(h, l, qhi_b[e]);
+  l += qlo_b[e];
+  qhi[e] = h;
+  qlo[e] = l;
+}
+
+__global__ void k_finalize(long n, double total, const float* __restrict__ mip, const int* __restrict__ bidx,
+                           const float* __restrict__ shi, const float* __restrict__ slo,
+                           const float* __restrict__ qhi, const float* __restrict__ qlo,
+                           const float* __restrict__ euler, int O, const float* __restrict__ dvals, int D,
+                           float* zs, float* mean_o, float* std_o, float* phi, float* theta, float* psi,
+                           float* df) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const double mean = ((double)shi[e] + (double)slo[e]) / total;
+  double var = ((double)qhi[e] + (double)qlo[e]) / total - mean * mean;
+  const double sd = sqrt(var > 0.0 ? var : 0.0);
+  const double z = sd != 0.0 ? ((double)mip[e] - mean) / sd : 0.0;
+  zs[e] = (float)z;
+  mean_o[e] = (float)mean;
+  std_o[e] = (float)sd;
+  const long stride = (long)D * O;
+  long rem = (long)bidx[e] % stride;
+  if (rem < 0) rem += stride;
+  const int d = (int)(rem / O), o = (int)(rem % O);
+  phi[e] = euler[3 * o];
+  theta[e] = euler[3 * o + 1];
+  psi[e] = euler[3 * o + 2];
+  df[e] = dvals[d];
+}
+
+}  // namespace tdtm
+
+// --------------------------------------------------------------------------- plan
+using namespace tdtm;
+
+struct tdtm_match_plan {
+  int H, W, h, B, D, C, K, G, Wr, wr;
+  float2* tw = nullptr;
+  float2* Xt = nullptr;
+  float* filt = nullptr;
+  float2* S = nullptr;
+  float2* Tb = nullptr;
+  float* P = nullptr;
+  float* part = nullptr;
+  float2* ae = nullptr;
+  float2* Rb = nullptr;
+  float2* Gb = nullptr;
+  int64_t bytes = 0;
+  bool image_set = false, filters_set = false;
+};
+
+template <class T>
+static int dalloc(T** p, size_t count, int64_t& acc) {
+  const size_t b = count * sizeof(T);
+  hipError_t e = hipMalloc((void**)p, b);
+  if (e != hipSuccess) return fail(std::string("hipMalloc(") + std::to_string(b) + "): " + hipGetErrorString(e));
+  acc += (int64_t)b;
+  return 0;
+}
+
+static void plan_free(tdtm_match_plan* p) {
+  void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+}
+
+// ---------------------------------------------------------- size dispatch helpers
+template <int HS>
+static void launch_proj(tdtm_match_plan* p, hipStream_t st) {
+  constexpr int CP = (HS / V) >= 64 ? 4 : 8;
+  constexpr int WR = HS / 2 + 1;
+  dim3 gc((WR + CP - 1) / CP, p->K);
+  k_proj_cols<HS, CP><<<gc, CP * (HS / V), 0, st>>>(p->S, p->filt, p->K / (p->D * p->C), p->D, p->tw, p->Tb);
+  constexpr int TRW = (HS / 2) / V;
+  constexpr int RPB0 = 256 / TRW;
+  constexpr int RPB = RPB0 > HS ? HS : RPB0;
+  dim3 gr(HS / RPB, p->K);
+  k_proj_rows<HS, RPB><<<gr, RPB * TRW, 0, st>>>(p->Tb, p->tw, p->P, p->part);
+}
+
+static int dispatch_proj(tdtm_match_plan* p, hipStream_t st) {
+  switch (p->h) {
+    case 32: launch_proj<32>(p, st); break;
+    case 64: launch_proj<64>(p, st); break;
+    case 128: launch_proj<128>(p, st); break;
+    case 256: launch_proj<256>(p, st); break;
+    case 512: launch_proj<512>(p, st); break;
+    default: return fail("unsupported template size");
+  }
+  return 0;
+}
+
+template <int WS, int NZJ>
+static void launch_rowfwd_n(tdtm_match_plan* p, int K, hipStream_t st) {
+  dim3 grid(p->h, K);
+  k_rowfwd<WS, NZJ><<<grid, (WS / 2) / V, 0, st>>>(p->P, p->ae, p->h, p->G, p->tw, p->Rb);
+}
+
+template <int WS>
+static void launch_rowfwd(tdtm_match_plan* p, int K, hipStream_t st) {
+  constexpr int T = (WS / 2) / V;
+  const int nz = (p->h / 2 + T - 1) / T;  // leading non-zero inputs per first-pass butterfly
+  if (nz <= 1) launch_rowfwd_n<WS, 1>(p, K, st);
+  else if (nz == 2) launch_rowfwd_n<WS, 2>(p, K, st);
+  else launch_rowfwd_n<WS, 8>(p, K, st);
+}
+
+template <int HH, int NZJ>
+static void launch_colpass_n(tdtm_match_plan* p, int K, hipStream_t st) {
+  int nchunk = (2048 + p->G - 1) / p->G;
+  if (nchunk > K) nchunk = K;
+  if (nchunk < 1) nchunk = 1;
+  const int kchunk = (K + nchunk - 1) / nchunk;
+  nchunk = (K + kchunk - 1) / kchunk;
+  dim3 grid(p->G, nchunk);
+  k_colpass<HH, NZJ><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, p->tw, p->Gb);
+}
+
+template <int HH>
+static void launch_colpass(tdtm_match_plan* p, int K, hipStream_t st) {
+  constexpr int T = HH / V;
+  const int nz = (p->h + T - 1) / T;
+  if (nz <= 1) launch_colpass_n<HH, 1>(p, K, st);
+  else if (nz == 2) launch_colpass_n<HH, 2>(p, K, st);
+  else launch_colpass_n<HH, 8>(p, K, st);
+}
+
+template <int WS>
+static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
+                           float* slo, float* qhi, float* qlo, hipStream_t st) {
+  dim3 grid(p->H / TR);
+  k_rowpass<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi,
+                                                    slo, qhi, qlo);
+}
+
+#define TDTM_SWITCH_POW2(val, FN, ...)             \
+  switch (val) {                                   \
+    case 64: FN<64>(__VA_ARGS__); break;           \
+    case 128: FN<128>(__VA_ARGS__); break;         \
+    case 256: FN<256>(__VA_ARGS__); break;         \
+    case 512: FN<512>(__VA_ARGS__); break;         \
+    case 1024: FN<1024>(__VA_ARGS__); break;       \
+    case 2048: FN<2048>(__VA_ARGS__); break;       \
+    case 4096: FN<4096>(__VA_ARGS__); break;       \
+    default: return fail("unsupported image size"); \
+  }
+
+extern "C" {
+
+int tdtm_abi_version(void) { return 1; }
+
+const char* tdtm_last_error(void) { return g_err.c_str(); }
+
+int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_plan_t* out) {
+  if (!out) return fail("out_plan is NULL");
+  *out = nullptr;
+  if (!is_pow2(H) || !is_pow2(W) || H < 64 || W < 64 || H > 4096 || W > 4096)
+    return fail("image size must be a power of two in [64, 4096], got " + std::to_string(H) + "x" +
+                std::to_string(W));
+  if (!is_pow2(h) || h < 32 || h > 512) return fail("template size must be a power of two in [32, 512]");
+  if (h > H || h > W) return fail("template larger than image");
+  if (B < 1 || D < 1 || C < 1) return fail("batch, defocus and pixel counts must be >= 1");
+  auto* p = new tdtm_match_plan();
+  p->H = H;
+  p->W = W;
+  p->h = h;
+  p->B = B;
+  p->D = D;
+  p->C = C;
+  p->K = B * D * C;
+  p->Wr = W / 2 + 1;
+  p->wr = h / 2 + 1;
+  p->G = (p->Wr + CW - 1) / CW;
+  const size_t K = p->K;
+  int rc = 0;
+  rc |= dalloc(&p->tw, TW_N, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->Xt, (size_t)p->G * H * CW, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->filt, (size_t)C * D * h * p->wr, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->S, (size_t)B * h * p->wr, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->Tb, K * h * p->wr, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->P, K * h * h, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->part, K * h * 3, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->ae, K, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->Rb, K * p->G * h * CW, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->G * CW, p->bytes);
+  if (rc) {
+    plan_free(p);
+    delete p;
+    return 1;
+  }
+  std::vector<float2> tw(TW_N);
+  for (int m = 0; m < TW_N; ++m) {
+    const double a = -2.0 * M_PI * (double)m / (double)TW_N;
+    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+  hipError_t e = hipMemcpy(p->tw, tw.data(), TW_N * sizeof(float2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    plan_free(p);
+    delete p;
+    return fail(std::string("twiddle upload: ") + hipGetErrorString(e));
+  }
+  *out = p;
+  return 0;
+}
+
+int tdtm_match_plan_destroy(tdtm_match_plan_t p) {
+  if (!p) return 0;
+  plan_free(p);
+  delete p;
+  return 0;
+}
+
+int64_t tdtm_match_plan_bytes(tdtm_match_plan_t p) { return p ? p->bytes : -1; }
+
+int tdtm_match_set_image(tdtm_match_plan_t p, const void* image_dft, void* stream) {
+  if (!p || !image_dft) return fail("null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const long total = (long)p->G * p->H * CW;
+  const float scale = (float)(1.0 / ((double)p->H * (double)p->W));
+  k_tile_image<<<(total + 255) / 256, 256, 0, st>>>((const float2*)image_dft, p->H, p->Wr, p->G, scale, p->Xt);
+  TDTM_LAUNCHED();
+  p->image_set = true;
+  return 0;
+}
+
+int tdtm_match_set_filters(tdtm_match_plan_t p, const float* ctf, const float* wf, void* stream) {
+  if (!p || !ctf || !wf) return fail("null argument");
+  const long plane = (long)p->h * p->wr;
+  const long total = plane * p->C * p->D;
+  k_filters<<<(total + 255) / 256, 256, 0, (hipStream_t)stream>>>(ctf, wf, plane, total, p->filt);
+  TDTM_LAUNCHED();
+  p->filters_set = true;
+  return 0;
+}
+
+int tdtm_stats_init(int64_t n, float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo,
+                    void* stream) {
+  if (n <= 0) return fail("n must be positive");
+  if (!mip || !idx || !shi || !slo || !qhi || !qlo) return fail("null argument");
+  k_stats_init<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, mip, idx, shi, slo, qhi, qlo);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+int tdtm_stats_merge(int64_t n, float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo,
+                     const float* mip_b, const int32_t* idx_b, const float* shi_b, const float* slo_b,
+                     const float* qhi_b, const float* qlo_b, void* stream) {
+  if (n <= 0) return fail("n must be positive");
+  k_stats_merge<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, mip, idx, shi, slo, qhi, qlo, mip_b, idx_b,
+                                                                 shi_b, slo_b, qhi_b, qlo_b);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* euler, int O, int start,
+                    int count, float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo,
+                    void* stream) {
+  if (!p || !template_dft || !euler || !mip || !idx || !shi || !slo || !qhi || !qlo) return fail("null argument");
+  if (!p->image_set || !p->filters_set) return fail("tdtm_match_set_image / set_filters not called");
+  if (count < 1 || count > p->B) return fail("count must be in [1, max_orientation_batch]");
+  if (start < 0 || start + count > O) return fail("orientation range out of bounds");
+  if ((long)p->C * p->D * O > 2147483647L) return fail("search space exceeds int32 global index");
+  hipStream_t st = (hipStream_t)stream;
+  const int K = count * p->D * p->C;
+  const long ns = (long)count * p->h * p->wr;
+  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, p->S);
+  TDTM_LAUNCHED();
+  // the projection kernels take their batch size from p->K; run them on exactly K
+  const int Ksave = p->K;
+  p->K = K;
+  int rc = dispatch_proj(p, st);
+  p->K = Ksave;
+  if (rc) return rc;
+  TDTM_LAUNCHED();
+  k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->H, p->W, p->ae);
+  TDTM_LAUNCHED();
+  TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
+  TDTM_LAUNCHED();
+  TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, st);
+  TDTM_LAUNCHED();
+  TDTM_SWITCH_POW2(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+int tdtm_match_finalize(int64_t n, int64_t total, const float* mip, const int32_t* idx, const float* shi,
+                        const float* slo, const float* qhi, const float* qlo, const float* euler, int O,
+                        const float* dvals, int D, float* zs, float* mean, float* sd, float* phi, float* theta,
+                        float* psi, float* df, void* stream) {
+  if (n <= 0 || total <= 0 || O < 1 || D < 1) return fail("bad sizes");
+  k_finalize<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, (double)total, mip, idx, shi, slo, qhi, qlo,
+                                                               euler, O, dvals, D, zs, mean, sd, phi, theta, psi,
+                                                               df);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+"""Whole-orientation search on the gfx950 HIP path.
+
+Drop-in for ``leopard_em.backend.core_match_template.core_match_template``
+(``src/leopard_em/backend/core_match_template.py:146-363``): same signature, same
+output dictionary.  Differences, all deliberate:
+
+* one process; every device in ``device`` gets a contiguous share of the
+  orientations and the partial statistics are merged on the first device with the
+  reference's rule (max, first device on ties; sums added) instead of the
+  multiprocessing work queue (``backend/distributed.py``).
+* sums and sums of squares are compensated (hi, lo) float32 pairs and the final
+  mean / std / z-score are computed in float64 on the device.
+* the best index of a partial last orientation batch is exact (the reference's
+  ``arange(i, i + batch)`` at ``core_match_template.py:513-518`` mis-maps it when
+  num_defocus * num_pixel > 1).
+* ``num_cuda_streams`` and ``backend`` are accepted for signature compatibility; all
+  work of a step is one fused launch sequence on the current stream.
+"""
+
+from __future__ import annotations
+
+import warnings
+
+import torch
+
+from .. import _lib
+
+
+class MatchPlan:
+    """Device plan + statistics maps for one search on one device."""
+
+    def __init__(self, image_h: int, image_w: int, template_h: int, orientation_batch_size: int,
+                 num_defocus: int, num_pixel: int, device: torch.device):
+        if device.type != "cuda":
+            raise _lib.TdtmError(f"the tdtm backend runs on a HIP device, got {device}")
+        self.lib = _lib.load()
+        self.device = device
+        self.H, self.W, self.h = image_h, image_w, template_h
+        self.B, self.D, self.C = orientation_batch_size, num_defocus, num_pixel
+        handle = _lib.c_void_p()
+        with torch.cuda.device(device):
+            _lib.check(self.lib.tdtm_match_plan_create(image_h, image_w, template_h, orientation_batch_size,
+                                                       num_defocus, num_pixel, _lib.ctypes.byref(handle)))
+        self.handle = handle
+        n = (image_h, image_w)
+        self.mip = torch.empty(n, dtype=torch.float32, device=device)
+        self.best_index = torch.empty(n, dtype=torch.int32, device=device)
+        self.sums = [torch.empty(n, dtype=torch.float32, device=device) for _ in range(4)]
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.tdtm_match_plan_destroy(h)
+            except Exception:  # noqa: BLE001 -- interpreter shutdown
+                pass
+            self.handle = None
+
+    @property
+    def bytes(self) -> int:
+        return int(self.lib.tdtm_match_plan_bytes(self.handle))
+
+    def _stream(self):
+        return _lib.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def set_inputs(self, image_dft: torch.Tensor, ctf_filters: torch.Tensor, whitening: torch.Tensor) -> None:
+        st = self._stream()
+        self._image = image_dft.to(self.device, torch.complex64).contiguous()
+        self._ctf = ctf_filters.to(self.device, torch.float32).contiguous()
+        self._wf = whitening.to(self.device, torch.float32).contiguous()
+        _lib.check(self.lib.tdtm_match_set_image(self.handle, _lib.ptr(self._image), st))
+        _lib.check(self.lib.tdtm_match_set_filters(self.handle, _lib.ptr(self._ctf), _lib.ptr(self._wf), st))
+
+    def reset_stats(self) -> None:
+        _lib.check(self.lib.tdtm_stats_init(self.H * self.W, _lib.ptr(self.mip), _lib.ptr(self.best_index),
+                                            *[_lib.ptr(s) for s in self.sums], self._stream()))
+
+    def step(self, template_dft: torch.Tensor, euler_deg: torch.Tensor, start: int, count: int) -> None:
+        _lib.check(self.lib.tdtm_match_step(
+            self.handle, _lib.ptr(template_dft), _lib.ptr(euler_deg), euler_deg.shape[0], start, count,
+            _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums], self._stream()))
+
+    def run(self, template_dft: torch.Tensor, euler_deg: torch.Tensor, start: int, end: int) -> None:
+        for i in range(start, end, self.B):
+            self.step(template_dft, euler_deg, i, min(self.B, end - i))
+
+    def merge_from(self, other: "MatchPlan") -> None:
+        """self.stats <- merge(self.stats, other.stats); other may live on another device."""
+        tens = [other.mip, other.best_index, *other.sums]
+        if other.device != self.device:
+            tens = [t.to(self.device) for t in tens]
+        _lib.check(self.lib.tdtm_stats_merge(
+            self.H * self.W, _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums],
+            *[_lib.ptr(t) for t in tens], self._stream()))
+        self._keep = tens
+
+    def finalize(self, euler_deg: torch.Tensor, defocus_values: torch.Tensor, total: int) -> dict:
+        dev = self.device
+        outs = [torch.empty((self.H, self.W), dtype=torch.float32, device=dev) for _ in range(7)]
+        _lib.check(self.lib.tdtm_match_finalize(
+            self.H * self.W, total, _lib.ptr(self.mip), _lib.ptr(self.best_index),
+            *[_lib.ptr(s) for s in self.sums], _lib.ptr(euler_deg), euler_deg.shape[0],
+            _lib.ptr(defocus_values), defocus_values.shape[0], *[_lib.ptr(o) for o in outs], self._stream()))
+        keys = ["scaled_mip", "correlation_mean", "correlation_variance", "best_phi", "best_theta", "best_psi",
+                "best_defocus"]
+        return dict(zip(keys, outs))
+
+
+def _as_device_list(device) -> list[torch.device]:
+    if isinstance(device, (list, tuple)):
+        devs = [torch.device(d) for d in device]
+    else:
+        devs = [torch.device(device)]
+    if not devs:
+        raise ValueError("no device given")
+    return devs
+
+
+# pylint: disable=too-many-arguments,too-many-locals
+def core_match_template(
+    image_dft: torch.Tensor,
+    template_dft: torch.Tensor,
+    ctf_filters: torch.Tensor,
+    whitening_filter_template: torch.Tensor,
+    defocus_values: torch.Tensor,
+    pixel_values: torch.Tensor,
+    euler_angles: torch.Tensor,
+    device: torch.device | list[torch.device],
+    orientation_batch_size: int = 1,
+    num_cuda_streams: int = 1,
+    backend: str = "streamed",
+    mag_matrix: torch.Tensor | None = None,
+) -> dict[str, torch.Tensor]:
+    """Whole-orientation 2DTM search; signature and outputs of the reference's
+    ``core_match_template`` (``backend/core_match_template.py:146-363``).
+
+    Returns CPU tensors: mip, scaled_mip, best_phi, best_theta, best_psi,
+    best_defocus, correlation_mean, correlation_variance (the standard deviation, as
+    in the reference), and the ints total_projections, total_orientations,
+    total_defocus.
+    """
+    _ = backend
+    if mag_matrix is not None:
+        raise NotImplementedError("anisotropic magnification (mag_matrix) is not supported by the tdtm backend")
+    total_cc_per_batch = orientation_batch_size * defocus_values.shape[0] * pixel_values.shape[0]
+    if num_cuda_streams > total_cc_per_batch:
+        warnings.warn(
+            f"Number of CUDA streams ({num_cuda_streams}) is greater than the number of "
+            f"cross-correlations per batch ({total_cc_per_batch}).", stacklevel=2)
+    devs = _as_device_list(device)
+    H = image_dft.shape[0]
+    W = 2 * (image_dft.shape[1] - 1)
+    h = template_dft.shape[1]
+    O, D, C = euler_angles.shape[0], defocus_values.shape[0], pixel_values.shape[0]
+    if ctf_filters.shape != (C, D, h, h // 2 + 1):
+        raise ValueError(f"ctf_filters shape {tuple(ctf_filters.shape)} != {(C, D, h, h // 2 + 1)}")
+    if whitening_filter_template.shape != (h, h // 2 + 1):
+        raise ValueError("whitening_filter_template must have shape (h, h//2+1)")
+    if template_dft.shape != (h, h, h // 2 + 1):
+        raise ValueError("template_dft must have shape (h, h, h//2+1)")
+    batch = max(1, min(orientation_batch_size, O))
+
+    plans = []
+    share = (O + len(devs) - 1) // len(devs)
+    for r, dev in enumerate(devs):
+        lo, hi = r * share, min(O, (r + 1) * share)
+        plan = MatchPlan(H, W, h, batch, D, C, dev)
+        with torch.cuda.device(dev):
+            plan.set_inputs(image_dft, ctf_filters, whitening_filter_template)
+            tdft = template_dft.to(dev, torch.complex64).contiguous()
+            eul = euler_angles.to(dev, torch.float32).contiguous()
+            plan.reset_stats()
+            if hi > lo:
+                plan.run(tdft, eul, lo, hi)
+            plan._tdft, plan._eul = tdft, eul
+        plans.append(plan)
+    main = plans[0]
+    with torch.cuda.device(main.device):
+        for other in plans[1:]:
+            torch.cuda.synchronize(other.device)
+            main.merge_from(other)
+        dvals = defocus_values.to(main.device, torch.float32).contiguous()
+        total = O * D * C
+        maps = main.finalize(main._eul, dvals, total)
+        torch.cuda.synchronize(main.device)
+    out = {
+        "mip": main.mip.cpu(),
+        "scaled_mip": maps["scaled_mip"].cpu(),
+        "best_phi": maps["best_phi"].cpu(),
+        "best_theta": maps["best_theta"].cpu(),
+        "best_psi": maps["best_psi"].cpu(),
+        "best_defocus": maps["best_defocus"].cpu(),
+        "correlation_mean": maps["correlation_mean"].cpu(),
+        "correlation_variance": maps["correlation_variance"].cpu(),
+        "total_projections": total,
+        "total_orientations": O,
+        "total_defocus": D,
+    }
+    return out
+
+
+def raw_statistics(plan: MatchPlan) -> dict[str, torch.Tensor]:
+    """Running maps of a plan as CPU tensors (sums as float64 hi+lo)."""
+    s = [t.double().cpu() for t in plan.sums]
+    return {
+        "mip": plan.mip.cpu(),
+        "best_global_index": plan.best_index.cpu(),
+        "correlation_sum": s[0] + s[1],
+        "correlation_squared_sum": s[2] + s[3],
+    }
