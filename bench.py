@@ -1,0 +1,182 @@
+"""2DTM search throughput on one MI355X: (orientation x defocus) correlations/s.
+
+Workload (production shape): 4096 x 4096 synthetic micrograph, 512^3 Gaussian-blob
+template (5000 blobs, sigma 1.5 px, radius 200 voxels, 1.0 A/px), 13 defocus planes
+(-1200..+1200 A step 200), orientation batches of --batch orientations.  A "step" is
+one orientation batch through the fused search (slice -> projection -> padded FFT
+correlation -> MIP / argmax / sum / sum-sq update); inputs are resident in HBM.
+
+Prints one JSON line (rank 0) with the throughput, an HBM roofline record for the
+fused step and a CPU baseline (the float32 CPU oracle on a bounded sample of the same
+workload, timed on this host).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "leopard-em_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FULL_GRID_ORIENTATIONS = 1_584_480  # 2.5 deg / 1.5 deg uniform grid (docs/benchmarks.md:20 ~1.58M)
+
+
+def algorithmic_bytes_per_correlation(H: int, W: int) -> int:
+    """One complex64 half spectrum written and read back between the 2D inverse FFT passes."""
+    return 2 * 8 * H * (W // 2 + 1)
+
+
+def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
+    import torch
+
+    from leopard_em_amd import synthetic
+    from leopard_em_amd.utils import preprocessing as pp
+
+    t = synthetic.make_blob_template(seed, h, n_blobs, radius)
+    vol = t.volume(device=device)
+    tdft = pp.volume_to_rfft_fourier_slice(vol).to(torch.complex64).contiguous()
+    mg = synthetic.make_micrograph(t, seed, (H, W), n_particles, 12200.0, 11800.0, 30.0, snr=0.05,
+                                   defocus_spread=3000.0, device=device)
+    img_dft, wf = pp.prepare_image(mg.image.to(device), h)
+    dfs = torch.arange(-1200.0, 1201.0, 200.0)
+    ctf = pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]), device=device)
+    return img_dft, tdft, ctf, wf.to(device), dfs, vol
+
+
+def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
+    """Float32 CPU oracle (reference algorithm) on n_orient orientations x all defocus."""
+    import torch
+
+    sys.path.insert(0, REPO)
+    from oracle import geometry as og
+    from oracle import search as osr
+
+    cores = os.cpu_count() or 1
+    torch.set_num_threads(cores)
+    X = img_dft.cpu()
+    T = tdft.cpu()
+    F = (ctf.cpu() * wf.cpu()[None, None])
+    e = eul[:n_orient].cpu()
+    # one warm-up correlation batch, then the timed sample
+    osr.batched_cross_correlate(X, T, og.euler_to_rotmat_zyz(e[:1], torch.float32), F[:, :1])
+    t0 = time.perf_counter()
+    n = 0
+    for i in range(0, n_orient, batch):
+        rot = og.euler_to_rotmat_zyz(e[i:i + batch], torch.float32)
+        cc = osr.batched_cross_correlate(X, T, rot, F)
+        n += cc.shape[0] * cc.shape[1] * cc.shape[2]
+        del cc
+    dt = time.perf_counter() - t0
+    return n / dt, cores, n, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=8, help="orientations per step")
+    ap.add_argument("--size", type=int, default=4096, help="image edge (H = W)")
+    ap.add_argument("--template", type=int, default=512)
+    ap.add_argument("--cpu-orientations", type=int, default=2, help="CPU baseline sample (x 13 defocus)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a design note, "
+              "not a measured configuration", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from leopard_em_amd.backend import MatchPlan
+
+    dev = torch.device("cuda:0")
+    H = W = args.size
+    h = args.template
+    print(f"[bench] building synthetic inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
+    n_blobs = 5000 if h == 512 else max(100, int(5000 * (h / 512) ** 3))
+    img_dft, tdft, ctf, wf, dfs, _ = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
+    D = dfs.shape[0]
+    B = args.batch
+    n_orient = (args.warmup + args.steps) * B
+    g = torch.Generator().manual_seed(1)
+    eul = (torch.rand((max(n_orient, 4096), 3), generator=g) * torch.tensor([360.0, 180.0, 360.0])).to(dev)
+
+    plan = MatchPlan(H, W, h, B, D, 1, dev)
+    plan.set_inputs(img_dft, ctf, wf)
+    plan.reset_stats()
+    stream = torch.cuda.current_stream(dev)
+    for i in range(args.warmup):
+        plan.step(tdft, eul, i * B, B)
+    torch.cuda.synchronize(dev)
+    print("[bench] warm-up done", file=sys.stderr, flush=True)
+
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        plan.step(tdft, eul, (args.warmup + i) * B, B)
+    ev1.record(stream)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    ms_per_step = gpu_ms / args.steps
+    corr = args.steps * B * D
+    value = corr / (gpu_ms / 1e3)
+    ab = algorithmic_bytes_per_correlation(H, W)
+    achieved = ab * B * D / (ms_per_step / 1e3) / 1e9
+    print(f"[bench] {args.steps} steps: {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms), {value:.1f} corr/s",
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        print("[bench] CPU baseline (float32 oracle) ...", file=sys.stderr, flush=True)
+        rate, cores, n, dt = cpu_baseline(img_dft, tdft, ctf, wf, eul, args.cpu_orientations, args.cpu_orientations)
+        cpu = {"value": rate, "unit": "correlations/s", "cores": cores, "kind": "port",
+               "sample": f"{args.cpu_orientations} orientations x {D} defocus = {n} correlations of "
+                         f"{H}x{W} x {h}^3 in {dt:.1f} s, torch CPU float32 oracle"}
+
+    line = {
+        "metric": "(orientation x defocus) correlations/s on 4096^2 image",
+        "value": value,
+        "unit": "correlations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded Gaussian-blob template and micrograph; random orientations)",
+        "config": {
+            "workload": f"plain 2DTM search, {H}x{W} micrograph x {h}^3 template, {D} defocus, "
+                        f"{B} orientations per step",
+            "image": [H, W], "template": h, "defocus": D, "orientation_batch": B,
+            "full_grid_hours_extrapolated": FULL_GRID_ORIENTATIONS * D / value / 3600.0,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "tdtm_match_step (fused slice..statistics launch sequence)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

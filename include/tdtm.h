@@ -105,6 +105,32 @@ int tdtm_stats_merge(int64_t n, float* mip_a, int32_t* idx_a, float* sum_hi_a, f
                      const float* sum_hi_b, const float* sum_lo_b, const float* sq_hi_b,
                      const float* sq_lo_b, void* stream);
 
+/*
+ * Local refinement of one particle: for every orientation offset (composed with the
+ * particle's pose as R(offset) @ R(pose)), defocus offset and pixel-size offset,
+ * correlate the template against the particle crop, z-score the valid region with
+ * the supplied mean / std maps and keep the best (first strict maximum over batches of
+ * `max_orientation_batch` offsets).  Replaces _core_refine_template_single_thread
+ * (src/leopard_em/backend/core_refine_template.py:555-767) including its per-particle
+ * CTF stack (utils/ctf_utils.py:19-116) and the angle composition / wrap of
+ * core_refine_template.py:504-536.  The plan is created with the crop size as image
+ * size, num_defocus = len(defocus_offsets), num_pixel = len(pixel_offsets).
+ *   particle_dft      complex64 (H, W/2+1), euler_pose float32 (3) on the device,
+ *   offsets           float32 (n_off, 3) degrees,  defocus_offsets (D), pixel_offsets (C),
+ *   ctf_params        host doubles: voltage kV, Cs mm, amplitude contrast, phase shift
+ *                     deg, pixel size A, B-factor A^2,
+ *   projective_filter float32 (h, h/2+1),  corr_mean / corr_std float32 (H-h+1, W-h+1),
+ *   out               float32[14] on the device: max_cc, max_z, phi, theta, psi (composed,
+ *                     wrapped), defocus offset, pixel-size offset, pos_y, pos_x (valid-crop
+ *                     coordinates), angle_idx, unused, phi/theta/psi offsets.
+ */
+int tdtm_refine_particle(tdtm_match_plan_t plan, const void* template_dft, const void* particle_dft,
+                         const float* euler_pose, const float* offsets, int n_off,
+                         const float* defocus_offsets, const float* pixel_offsets, double defocus_u,
+                         double defocus_v, double defocus_angle, const double* ctf_params,
+                         const float* projective_filter, const float* corr_mean, const float* corr_std,
+                         float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

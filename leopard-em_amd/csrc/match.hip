@@ -76,8 +76,27 @@ __global__ void k_filters(const float* __restrict__ ctf, const float* __restrict
 }
 
 // Trilinear central slice of the fftshifted 3D rfft; output unshifted in y.
+__device__ __forceinline__ void rotmat_zyz(const float* e, float (&r)[9]) {
+  const float d2r = 0.017453292519943295f;
+  float sph, cph, sth, cth, sps, cps;
+  sincosf(e[0] * d2r, &sph, &cph);
+  sincosf(e[1] * d2r, &sth, &cth);
+  sincosf(e[2] * d2r, &sps, &cps);
+  r[0] = cph * cth * cps - sph * sps;
+  r[1] = -cph * cth * sps - sph * cps;
+  r[2] = cph * sth;
+  r[3] = sph * cth * cps + cph * sps;
+  r[4] = -sph * cth * sps + cph * cps;
+  r[5] = sph * sth;
+  r[6] = -sth * cps;
+  r[7] = sth * sps;
+  r[8] = cth;
+}
+
+// R = Rz(phi) Ry(theta) Rz(psi) of orientation o0 + b, right-multiplied by the rotation
+// of `base` (refinement: offset composed with the particle's pose) when base != nullptr.
 __global__ void k_slice(const float2* __restrict__ vol, int n, const float* __restrict__ euler, int o0, int nb,
-                        float2* __restrict__ S) {
+                        const float* __restrict__ base, float2* __restrict__ S) {
   const int wr = n / 2 + 1;
   const long total = (long)nb * n * wr;
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -85,15 +104,19 @@ __global__ void k_slice(const float2* __restrict__ vol, int n, const float* __re
   const int kx = idx % wr;
   const int r = (idx / wr) % n;
   const int b = idx / ((long)wr * n);
-  const float d2r = 0.017453292519943295f;
-  float sph, cph, sth, cth, sps, cps;
-  sincosf(euler[3 * (o0 + b) + 0] * d2r, &sph, &cph);
-  sincosf(euler[3 * (o0 + b) + 1] * d2r, &sth, &cth);
-  sincosf(euler[3 * (o0 + b) + 2] * d2r, &sps, &cps);
-  // columns 0 and 1 of R = Rz(phi) Ry(theta) Rz(psi)
-  const float r00 = cph * cth * cps - sph * sps, r01 = -cph * cth * sps - sph * cps;
-  const float r10 = sph * cth * cps + cph * sps, r11 = -sph * cth * sps + cph * cps;
-  const float r20 = -sth * cps, r21 = sth * sps;
+  float ro[9];
+  rotmat_zyz(euler + 3 * (o0 + b), ro);
+  if (base != nullptr) {
+    float rb[9], rc[9];
+    rotmat_zyz(base, rb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) rc[3 * i + j] = ro[3 * i] * rb[j] + ro[3 * i + 1] * rb[3 + j] + ro[3 * i + 2] * rb[6 + j];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ro[i] = rc[i];
+  }
+  const float r00 = ro[0], r01 = ro[1], r10 = ro[3], r11 = ro[4], r20 = ro[6], r21 = ro[7];
   const float fx = (float)kx / (float)n;
   const float fy = (float)(r < n / 2 ? r : r - n) / (float)n;
   float rx = r00 * fx + r01 * fy, ry = r10 * fx + r11 * fy, rz = r20 * fx + r21 * fy;
@@ -318,7 +341,7 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
 // conjugate against the image spectrum column, inverse FFT along y; tiled store.
 template <int HH, int NZJ>
 __global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
-                                                        int hs, int G, int K, int kchunk,
+                                                        int hs, int G, int K, int kchunk, int ystore,
                                                         const float2* __restrict__ tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr int LF = LdsSize<HH>::floats;
@@ -350,7 +373,7 @@ __global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restric
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int yy = t + i * T;
-      gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+      if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
     }
   }
 }
@@ -523,6 +546,260 @@ __global__ void k_finalize(long n, double total, const float* __restrict__ mip, 
   df[e] = dvals[d];
 }
 
+
+// ------------------------------------------------------------------ refinement
+// CTF x B-envelope stack for one particle: (C, D, h, h/2+1), torch_ctf sign convention.
+__global__ void k_ctf_stack(int h, int D, int C, double df_mean, double astig, double astig_ang,
+                            const float* __restrict__ dfo, const float* __restrict__ pxo, double lam, double cs_a,
+                            double px, double wac, double phase, double bfac, float* __restrict__ out) {
+  const int wr = h / 2 + 1;
+  const long plane = (long)h * wr;
+  const long total = plane * D * C;
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= total) return;
+  const int kx = e % wr;
+  const int r = (e / wr) % h;
+  const int d = (e / plane) % D;
+  const int c = e / (plane * D);
+  const double ky = (double)(r < (h + 1) / 2 ? r : r - h) / (double)h / px;
+  const double kxx = (double)kx / (double)h / px;
+  const double k2 = ky * ky + kxx * kxx;
+  const double ang = atan2(ky, kxx);
+  const double ratio = (px + (double)pxo[c]) / px;
+  const double cs = cs_a / (ratio * ratio * ratio * ratio);
+  const double df = (double)dfo[d] + df_mean + astig * cos(2.0 * (ang - astig_ang));
+  const double chi = -M_PI * lam * k2 * df + 0.5 * M_PI * cs * lam * lam * lam * k2 * k2 - wac - phase;
+  out[e] = (float)(-sin(chi) * exp(-bfac * k2 / 4.0));
+}
+
+struct RefineBest {
+  float z;
+  float cc;
+  int pix;
+  int k;
+};
+
+// better candidate: larger z; equal z -> smaller flat pixel index (torch.max, first)
+__device__ __forceinline__ bool ref_better(float z, int pix, float zb, int pixb) {
+  return z > zb || (z == zb && pix < pixb);
+}
+
+// Row pass of a refinement batch: c2r along x for the rows of the valid region and the
+// z-score epilogue z = (cc - mean) / std on the valid crop.  Per pixel the first
+// correlation reaching the pixel's maximum z is kept (torch.max over the flattened
+// (pixel size, defocus, offset) axis); each block then reduces to its best
+// (z, pixel, correlation) and its maximum raw cc.
+template <int WS>
+__global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2* __restrict__ Gb, int HH, int G,
+                                                                   int K, int crop_h, int crop_w,
+                                                                   const float* __restrict__ mean,
+                                                                   const float* __restrict__ stdv,
+                                                                   const float2* __restrict__ tw,
+                                                                   RefineBest* __restrict__ part) {
+  constexpr int M = WS / 2;
+  constexpr int T = M / V;
+  constexpr int LF = LdsSize<M>::floats;
+  __shared__ float lds[TR * 2 * LF];
+  __shared__ RefineBest red[TR * T];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  float* lre = lds + r * 2 * LF;
+  float* lim = lre + LF;
+  const int p = blockIdx.x;
+  const int y = p * TR + r;
+  float mu[2 * V], sd[2 * V], zb[2 * V];
+  int kb[2 * V];
+  float ccmax = -INFINITY;
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int x = 2 * (t + i * T) + hh;
+      const bool ok = y < crop_h && x < crop_w;
+      mu[2 * i + hh] = ok ? mean[(long)y * crop_w + x] : 0.f;
+      sd[2 * i + hh] = ok ? stdv[(long)y * crop_w + x] : 1.f;
+      zb[2 * i + hh] = -INFINITY;
+      kb[2 * i + hh] = -1;
+    }
+  const int tile_elems = G * TR * CW;
+  for (int k = 0; k < K; ++k) {
+    const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
+    for (int e = threadIdx.x; e < tile_elems; e += blockDim.x) {
+      const int cc = e % CW, rr = (e / CW) % TR, gg = e / (CW * TR);
+      const int kx = gg * CW + cc;
+      if (kx <= M) {
+        const float2 val = gin[e];
+        const int pp = lds_pad(kx);
+        lds[rr * 2 * LF + pp] = val.x;
+        lds[rr * 2 * LF + LF + pp] = val.y;
+      }
+    }
+    __syncthreads();
+    float2 v[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int mm = t + i * T;
+      float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
+      const int bm = M - mm;
+      float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
+      if (mm == 0) {
+        A.y = 0.f;
+        Bc.y = 0.f;
+      }
+      const float2 w = c_conj(tw[mm * (TW_N / WS)]);
+      const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
+      v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
+    }
+    __syncthreads();
+    fft_wg<M, V, true>(v, lre, lim, t, tw);
+#pragma unroll
+    for (int i = 0; i < V; ++i)
+#pragma unroll
+      for (int hh = 0; hh < 2; ++hh) {
+        const int x = 2 * (t + i * T) + hh;
+        if (y < crop_h && x < crop_w) {
+          const float c = hh ? v[i].y : v[i].x;
+          ccmax = fmaxf(ccmax, c);
+          const float z = (c - mu[2 * i + hh]) / sd[2 * i + hh];
+          if (z > zb[2 * i + hh]) {
+            zb[2 * i + hh] = z;
+            kb[2 * i + hh] = k;
+          }
+        }
+      }
+  }
+  RefineBest b{-INFINITY, ccmax, 0x7fffffff, -1};
+#pragma unroll
+  for (int i = 0; i < V; ++i)
+#pragma unroll
+    for (int hh = 0; hh < 2; ++hh) {
+      const int x = 2 * (t + i * T) + hh;
+      if (y < crop_h && x < crop_w && kb[2 * i + hh] >= 0) {
+        const int pix = y * crop_w + x;
+        if (ref_better(zb[2 * i + hh], pix, b.z, b.pix)) {
+          b.z = zb[2 * i + hh];
+          b.pix = pix;
+          b.k = kb[2 * i + hh];
+        }
+      }
+    }
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      RefineBest o = red[threadIdx.x + off];
+      RefineBest s = red[threadIdx.x];
+      s.cc = fmaxf(s.cc, o.cc);
+      if (ref_better(o.z, o.pix, s.z, s.pix)) {
+        s.z = o.z;
+        s.pix = o.pix;
+        s.k = o.k;
+      }
+      red[threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Reduce the row-pass blocks of one batch into its slot.
+__global__ void k_refine_reduce(const RefineBest* __restrict__ part, int n, RefineBest* __restrict__ slot) {
+  __shared__ RefineBest red[256];
+  RefineBest b{-INFINITY, -INFINITY, 0x7fffffff, -1};
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    RefineBest o = part[i];
+    b.cc = fmaxf(b.cc, o.cc);
+    if (ref_better(o.z, o.pix, b.z, b.pix)) {
+      b.z = o.z;
+      b.pix = o.pix;
+      b.k = o.k;
+    }
+  }
+  red[threadIdx.x] = b;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      RefineBest o = red[threadIdx.x + off];
+      RefineBest s = red[threadIdx.x];
+      s.cc = fmaxf(s.cc, o.cc);
+      if (ref_better(o.z, o.pix, s.z, s.pix)) {
+        s.z = o.z;
+        s.pix = o.pix;
+        s.k = o.k;
+      }
+      red[threadIdx.x] = s;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *slot = red[0];
+}
+
+// Pick the first batch whose z maximum strictly beats the running best
+// (core_refine_template.py:725) and decode it; compose the offset with the pose and
+// wrap the angles (core_refine_template.py:504-536).  out[11]: max_cc, max_z, phi,
+// theta, psi (composed, wrapped), defocus offset, pixel-size offset, pos_y, pos_x
+// (valid-crop coordinates), angle_idx, and the phi/theta/psi offsets in out[11..13].
+__global__ void k_refine_select(const RefineBest* __restrict__ slots, int nbatch, int batch, int n_off, int D,
+                                const float* __restrict__ off, const float* __restrict__ base,
+                                const float* __restrict__ dfo, const float* __restrict__ pxo, int crop_w,
+                                float* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  float best = -1e9f;
+  int bi = -1;
+  for (int i = 0; i < nbatch; ++i)
+    if (slots[i].z > best) {
+      best = slots[i].z;
+      bi = i;
+    }
+  float res[14] = {-1e9f, -1e9f, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  float phio = 0.f, theo = 0.f, psio = 0.f;
+  if (bi >= 0) {
+    const RefineBest s = slots[bi];
+    const int s0 = bi * batch;
+    const int nb = min(batch, n_off - s0);
+    const int f = s.k;
+    const int px = f / (D * nb), d = (f / nb) % D, a = f % nb;
+    res[0] = s.cc;
+    res[1] = s.z;
+    phio = off[3 * (s0 + a)];
+    theo = off[3 * (s0 + a) + 1];
+    psio = off[3 * (s0 + a) + 2];
+    res[5] = dfo[d];
+    res[6] = pxo[px];
+    res[7] = (float)(s.pix / crop_w);
+    res[8] = (float)(s.pix % crop_w);
+    res[9] = (float)(s0 + a);
+  }
+  // compose R(offset) @ R(pose) and convert back to ZYZ degrees
+  float ro[9], rb[9], rc[9];
+  const float oe[3] = {phio, theo, psio};
+  rotmat_zyz(oe, ro);
+  rotmat_zyz(base, rb);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) rc[3 * i + j] = ro[3 * i] * rb[j] + ro[3 * i + 1] * rb[3 + j] + ro[3 * i + 2] * rb[6 + j];
+  const double r22 = fmin(1.0, fmax(-1.0, (double)rc[8]));
+  double theta = acos(r22);
+  double phi, psi;
+  if (fabs(sin(theta)) < 1e-7) {
+    phi = r22 > 0 ? atan2((double)rc[3], (double)rc[0]) : atan2(-(double)rc[3], -(double)rc[0]);
+    psi = 0.0;
+  } else {
+    phi = atan2((double)rc[5], (double)rc[2]);
+    psi = atan2((double)rc[7], -(double)rc[6]);
+  }
+  const double r2d = 57.29577951308232;
+  float p = (float)(phi * r2d), th = (float)(theta * r2d), ps = (float)(psi * r2d);
+  if (p < 0) p += 360.f;
+  if (th < 0) th += 180.f;
+  if (ps < 0) ps += 360.f;
+  res[2] = p;
+  res[3] = th;
+  res[4] = ps;
+  res[11] = phio;
+  res[12] = theo;
+  res[13] = psio;
+  for (int i = 0; i < 14; ++i) out[i] = res[i];
+}
+
 }  // namespace tdtm
 
 // --------------------------------------------------------------------------- plan
@@ -540,6 +817,10 @@ struct tdtm_match_plan {
   float2* ae = nullptr;
   float2* Rb = nullptr;
   float2* Gb = nullptr;
+  float* ctfbuf = nullptr;
+  RefineBest* rpart = nullptr;
+  RefineBest* rslots = nullptr;
+  int rslots_cap = 0;
   int64_t bytes = 0;
   bool image_set = false, filters_set = false;
 };
@@ -554,7 +835,8 @@ static int dalloc(T** p, size_t count, int64_t& acc) {
 }
 
 static void plan_free(tdtm_match_plan* p) {
-  void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb};
+  void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb, p->ctfbuf, p->rpart,
+                  p->rslots};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
 }
@@ -601,23 +883,23 @@ static void launch_rowfwd(tdtm_match_plan* p, int K, hipStream_t st) {
 }
 
 template <int HH, int NZJ>
-static void launch_colpass_n(tdtm_match_plan* p, int K, hipStream_t st) {
+static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t st) {
   int nchunk = (2048 + p->G - 1) / p->G;
   if (nchunk > K) nchunk = K;
   if (nchunk < 1) nchunk = 1;
   const int kchunk = (K + nchunk - 1) / nchunk;
   nchunk = (K + kchunk - 1) / kchunk;
   dim3 grid(p->G, nchunk);
-  k_colpass<HH, NZJ><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, p->tw, p->Gb);
+  k_colpass<HH, NZJ><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
 }
 
 template <int HH>
-static void launch_colpass(tdtm_match_plan* p, int K, hipStream_t st) {
+static void launch_colpass(tdtm_match_plan* p, int K, int ystore, hipStream_t st) {
   constexpr int T = HH / V;
   const int nz = (p->h + T - 1) / T;
-  if (nz <= 1) launch_colpass_n<HH, 1>(p, K, st);
-  else if (nz == 2) launch_colpass_n<HH, 2>(p, K, st);
-  else launch_colpass_n<HH, 8>(p, K, st);
+  if (nz <= 1) launch_colpass_n<HH, 1>(p, K, ystore, st);
+  else if (nz == 2) launch_colpass_n<HH, 2>(p, K, ystore, st);
+  else launch_colpass_n<HH, 8>(p, K, ystore, st);
 }
 
 template <int WS>
@@ -626,6 +908,14 @@ static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, flo
   dim3 grid(p->H / TR);
   k_rowpass<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi,
                                                     slo, qhi, qlo);
+}
+
+template <int WS>
+static void launch_rowpass_refine(tdtm_match_plan* p, int K, int crop_h, int crop_w, const float* mean,
+                                  const float* sd, hipStream_t st) {
+  dim3 grid((crop_h + TR - 1) / TR);
+  k_rowpass_refine<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, crop_h, crop_w, mean, sd, p->tw,
+                                                           p->rpart);
 }
 
 #define TDTM_SWITCH_POW2(val, FN, ...)             \
@@ -678,6 +968,8 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   rc |= rc ? 0 : dalloc(&p->ae, K, p->bytes);
   rc |= rc ? 0 : dalloc(&p->Rb, K * p->G * h * CW, p->bytes);
   rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->G * CW, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->ctfbuf, (size_t)C * D * h * p->wr, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR, p->bytes);
   if (rc) {
     plan_free(p);
     delete p;
@@ -758,7 +1050,7 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   hipStream_t st = (hipStream_t)stream;
   const int K = count * p->D * p->C;
   const long ns = (long)count * p->h * p->wr;
-  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, p->S);
+  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S);
   TDTM_LAUNCHED();
   // the projection kernels take their batch size from p->K; run them on exactly K
   const int Ksave = p->K;
@@ -771,7 +1063,7 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   TDTM_LAUNCHED();
   TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
   TDTM_LAUNCHED();
-  TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, st);
+  TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, p->H, st);
   TDTM_LAUNCHED();
   TDTM_SWITCH_POW2(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
   TDTM_LAUNCHED();
@@ -786,6 +1078,78 @@ int tdtm_match_finalize(int64_t n, int64_t total, const float* mip, const int32_
   k_finalize<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, (double)total, mip, idx, shi, slo, qhi, qlo,
                                                                euler, O, dvals, D, zs, mean, sd, phi, theta, psi,
                                                                df);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const void* particle_dft,
+                         const float* euler_pose, const float* offsets, int n_off, const float* defocus_offsets,
+                         const float* pixel_offsets, double defocus_u, double defocus_v, double defocus_angle,
+                         const double* ctf_params, const float* projective_filter, const float* corr_mean,
+                         const float* corr_std, float* out, void* stream) {
+  if (!p || !template_dft || !particle_dft || !euler_pose || !offsets || !defocus_offsets || !pixel_offsets ||
+      !ctf_params || !projective_filter || !corr_mean || !corr_std || !out)
+    return fail("null argument");
+  if (n_off < 1) return fail("no orientation offsets");
+  hipStream_t st = (hipStream_t)stream;
+  const int crop_h = p->H - p->h + 1, crop_w = p->W - p->h + 1;
+  const int nbatch = (n_off + p->B - 1) / p->B;
+  if (nbatch > p->rslots_cap) {
+    if (p->rslots) TDTM_HIP(hipFree(p->rslots));
+    p->rslots = nullptr;
+    p->rslots_cap = 0;
+    if (dalloc(&p->rslots, (size_t)nbatch, p->bytes)) return 1;
+    p->rslots_cap = nbatch;
+  }
+  // image tiles (match plan semantics: scale 1/(H W))
+  if (tdtm_match_set_image(p, particle_dft, stream)) return 1;
+  // per-particle CTF stack and projective filter
+  const double voltage = ctf_params[0], cs_mm = ctf_params[1], ac = ctf_params[2], phase = ctf_params[3];
+  const double px = ctf_params[4], bfac = ctf_params[5];
+  const double v = voltage * 1e3;
+  const double lam = 6.62607015e-34 /
+                     std::sqrt(2 * 9.1093837015e-31 * 1.602176634e-19 * v *
+                               (1 + 1.602176634e-19 * v / (2 * 9.1093837015e-31 * 299792458.0 * 299792458.0))) *
+                     1e10;
+  const long plane = (long)p->h * p->wr;
+  const long tot = plane * p->D * p->C;
+  k_ctf_stack<<<(tot + 255) / 256, 256, 0, st>>>(p->h, p->D, p->C, 0.5 * (defocus_u + defocus_v),
+                                                 0.5 * std::fabs(defocus_u - defocus_v), defocus_angle * M_PI / 180.0,
+                                                 defocus_offsets, pixel_offsets, lam, cs_mm * 1e7, px,
+                                                 std::atan(ac / std::sqrt(1 - ac * ac)), phase * M_PI / 180.0, bfac,
+                                                 p->ctfbuf);
+  TDTM_LAUNCHED();
+  k_filters<<<(tot + 255) / 256, 256, 0, st>>>(p->ctfbuf, projective_filter, plane, tot, p->filt);
+  TDTM_LAUNCHED();
+  const int ystore = ((crop_h + TR - 1) / TR) * TR;
+  const int nparts = (crop_h + TR - 1) / TR;
+  for (int bi = 0; bi < nbatch; ++bi) {
+    const int start = bi * p->B;
+    const int count = std::min(p->B, n_off - start);
+    const int K = count * p->D * p->C;
+    const long ns = (long)count * p->h * p->wr;
+    k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, offsets, start, count, euler_pose,
+                                               p->S);
+    TDTM_LAUNCHED();
+    const int Ksave = p->K;
+    p->K = K;
+    int rc = dispatch_proj(p, st);
+    p->K = Ksave;
+    if (rc) return rc;
+    TDTM_LAUNCHED();
+    k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->H, p->W, p->ae);
+    TDTM_LAUNCHED();
+    TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
+    TDTM_LAUNCHED();
+    TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, ystore, st);
+    TDTM_LAUNCHED();
+    TDTM_SWITCH_POW2(p->W, launch_rowpass_refine, p, K, crop_h, crop_w, corr_mean, corr_std, st);
+    TDTM_LAUNCHED();
+    k_refine_reduce<<<1, 256, 0, st>>>(p->rpart, nparts, p->rslots + bi);
+    TDTM_LAUNCHED();
+  }
+  k_refine_select<<<1, 64, 0, st>>>(p->rslots, nbatch, p->B, n_off, p->D, offsets, euler_pose, defocus_offsets,
+                                     pixel_offsets, crop_w, out);
   TDTM_LAUNCHED();
   return 0;
 }

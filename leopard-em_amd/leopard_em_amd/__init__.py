@@ -5,6 +5,6 @@ Drop-in counterparts of ``leopard_em.backend.core_match_template`` and
 C ABI in ``include/tdtm.h``.
 """
 
-from .backend import core_match_template  # noqa: F401
+from .backend import core_match_template, core_refine_template  # noqa: F401
 
-__all__ = ["core_match_template"]
+__all__ = ["core_match_template", "core_refine_template"]

@@ -32,6 +32,9 @@ SIGNATURES = {
     "tdtm_match_finalize": (c_int, [c_int64, c_int64] + [c_void_p] * 6 + [c_void_p, c_int, c_void_p, c_int]
                             + [c_void_p] * 7 + [c_void_p]),
     "tdtm_stats_merge": (c_int, [c_int64] + [c_void_p] * 12 + [c_void_p]),
+    "tdtm_refine_particle": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
+                                     c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 

@@ -1,0 +1,131 @@
+"""Once-per-search input preparation (the manager side of the 2DTM boundary).
+
+Counterparts of what ``MatchTemplateManager.make_backend_core_function_kwargs``
+(``pydantic_models/managers/match_template_manager.py:140-218``) builds before it calls
+``core_match_template``: the pre-processed image rfft, the template-grid whitening
+filter, the CTF filter stack and the fftshifted 3D template rfft.  These run once per
+micrograph on torch tensors (CPU or GPU) and are not part of the timed search.
+
+* ``preprocess_image``            ``utils/image_processing.py:16-75``
+* ``volume_to_rfft_fourier_slice`` ``utils/image_processing.py:179-202``
+* ``whitening_filter``            ``config/correlation_filters.py:49-108`` (radial PSD^-1/2)
+* ``ctf_filter_stack``            ``utils/ctf_utils.py:19-116`` (torch_ctf sign convention,
+                                  CTF(0) = +amplitude contrast, B-envelope)
+"""
+
+from __future__ import annotations
+
+import math
+
+import torch
+
+_H = 6.62607015e-34
+_ME = 9.1093837015e-31
+_E = 1.602176634e-19
+_C = 299792458.0
+
+
+def electron_wavelength_angstrom(voltage_kv: float) -> float:
+    v = voltage_kv * 1e3
+    return _H / math.sqrt(2 * _ME * _E * v * (1 + _E * v / (2 * _ME * _C**2))) * 1e10
+
+
+def _rfft_grid(shape, device, dtype):
+    h, w = shape
+    ky = torch.fft.fftfreq(h, device=device, dtype=dtype)
+    kx = torch.fft.rfftfreq(w, device=device, dtype=dtype)
+    return torch.meshgrid(ky, kx, indexing="ij")
+
+
+def ctf_filter_stack(template_shape, defocus_u, defocus_v, astigmatism_angle, defocus_offsets,
+                     pixel_size_offsets, *, voltage=300.0, spherical_aberration=2.7,
+                     amplitude_contrast_ratio=0.07, phase_shift=0.0, pixel_size=1.0, ctf_B_factor=0.0,
+                     device="cpu") -> torch.Tensor:
+    """(num_pixel_sizes, num_defocus, h, w//2+1) float32 CTF x B-envelope stack.
+
+    defocus = offsets + (u+v)/2, astigmatism |u-v|/2 at ``astigmatism_angle`` degrees,
+    Cs scaled per pixel size as Cs / (p/p0)^4 (``utils/search_utils.py:27-28``).
+    """
+    dt = torch.float64
+    dfs = torch.as_tensor(defocus_offsets, dtype=dt, device=device) + 0.5 * (defocus_u + defocus_v)
+    astig = abs(defocus_u - defocus_v) / 2
+    px_off = torch.as_tensor(pixel_size_offsets, dtype=dt, device=device)
+    cs_vals = spherical_aberration / ((pixel_size + px_off) / pixel_size) ** 4
+    lam = electron_wavelength_angstrom(voltage)
+    ky, kx = _rfft_grid(template_shape, device, dt)
+    ky, kx = ky / pixel_size, kx / pixel_size
+    k2 = ky**2 + kx**2
+    ang = torch.atan2(ky, kx)
+    w = math.atan(amplitude_contrast_ratio / math.sqrt(1 - amplitude_contrast_ratio**2))
+    ph = math.radians(phase_shift)
+    env = torch.exp(-ctf_B_factor * k2 / 4)
+    planes = []
+    for cs in cs_vals.tolist():
+        df = dfs[:, None, None] + astig * torch.cos(2 * (ang - math.radians(astigmatism_angle)))[None]
+        chi = -math.pi * lam * k2[None] * df + math.pi / 2 * cs * 1e7 * lam**3 * k2[None] ** 2 - w - ph
+        planes.append(-torch.sin(chi) * env[None])
+    return torch.stack(planes, 0).to(torch.float32)
+
+
+def whitening_filter(image_dft: torch.Tensor, output_shape_rfft, num_freq_bins=None, max_freq=0.5) -> torch.Tensor:
+    """1/sqrt(radially averaged |X|^2), max-normalised, resampled to an rfft grid (float32)."""
+    dt = torch.float64
+    dev = image_dft.device
+    hh, wr = image_dft.shape
+    ww = 2 * (wr - 1)
+    nb = num_freq_bins or (min(hh, ww) // 2 + 1)
+    ky, kx = _rfft_grid((hh, ww), dev, dt)
+    fr = torch.sqrt(ky**2 + kx**2)
+    power = image_dft.abs().to(dt) ** 2
+    idx = torch.clamp((fr / max_freq * (nb - 1)).round().long(), 0, nb - 1)
+    m = fr <= max_freq
+    sums = torch.zeros(nb, dtype=dt, device=dev).index_add_(0, idx[m], power[m])
+    cnt = torch.zeros(nb, dtype=dt, device=dev).index_add_(0, idx[m], torch.ones_like(power[m]))
+    avg = sums / cnt.clamp_min(1)
+    avg = torch.where(avg > 0, avg, torch.full_like(avg, float("inf")))
+    curve = avg.rsqrt()
+    curve = curve / curve.max()
+    h, w2 = output_shape_rfft
+    oy, ox = _rfft_grid((h, 2 * (w2 - 1)), dev, dt)
+    of = torch.sqrt(oy**2 + ox**2)
+    pos = of / max_freq * (nb - 1)
+    i0 = torch.clamp(pos.floor().long(), 0, nb - 1)
+    i1 = torch.clamp(i0 + 1, 0, nb - 1)
+    t = pos - i0.to(dt)
+    out = curve[i0] * (1 - t) + curve[i1] * t
+    return torch.where(of <= max_freq, out, torch.ones_like(out)).to(torch.float32)
+
+
+def preprocess_image(image_rfft, cumulative_fourier_filters, bandpass_filter, full_image_shape,
+                     extracted_box_shape) -> torch.Tensor:
+    """Filter, normalise to unit real-space variance, scale by sqrt(dimensionality)."""
+    x = image_rfft * cumulative_fourier_filters
+    sq = torch.abs(x) ** 2
+    ssum = sq.sum(dim=(-2, -1), keepdim=True) + sq[..., :, 1:-1].sum(dim=(-2, -1), keepdim=True)
+    x = x / torch.sqrt(ssum)
+    x = x * (bandpass_filter.sum() + bandpass_filter[:, 1:-1].sum()) ** 0.5
+    ih, iw = full_image_shape
+    bh, bw = extracted_box_shape
+    return x * ((ih * iw) / (bh * bw)) ** 0.5
+
+
+def volume_to_rfft_fourier_slice(volume: torch.Tensor) -> torch.Tensor:
+    """fftshift -> rfftn -> fftshift(dims 0, 1) of a cubic volume."""
+    v = torch.fft.fftshift(volume, dim=(0, 1, 2))
+    v = torch.fft.rfftn(v, dim=(0, 1, 2))
+    return torch.fft.fftshift(v, dim=(0, 1))
+
+
+def prepare_image(image: torch.Tensor, template_size: int):
+    """(pre-processed image rfft complex64, template-grid whitening filter float32).
+
+    Default filter configuration of the reference (whitening on, bandpass and
+    arbitrary-curve filters off).
+    """
+    img_dft = torch.fft.rfftn(image.to(torch.float32))
+    img_dft[0, 0] = 0
+    wf_img = whitening_filter(img_dft, tuple(img_dft.shape))
+    wf_tmp = whitening_filter(img_dft, (template_size, template_size // 2 + 1))
+    bp = torch.ones(img_dft.shape, dtype=torch.float32, device=img_dft.device)
+    pre = preprocess_image(img_dft, wf_img, bp, tuple(image.shape), tuple(image.shape))
+    return pre.to(torch.complex64), wf_tmp

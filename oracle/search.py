@@ -184,9 +184,12 @@ def refine_particle(particle_image_dft, template_dft, euler_angle, euler_angle_o
                 refined_pos_y=0, refined_pos_x=0, angle_idx=0)
     rdt = torch.float64 if dtype == torch.float64 else torch.float32
     default_rot = euler_to_rotmat_zyz(euler_angle, rdt)
+    kw = {k: v for k, v in ctf_kwargs.items()
+          if k in ("voltage", "spherical_aberration", "amplitude_contrast_ratio", "phase_shift", "pixel_size",
+                   "ctf_B_factor")}
     ctf = calculate_ctf_filter_stack_full_args(
         (th, tw), float(defocus_u), float(defocus_v), defocus_offsets, pixel_size_offsets,
-        astigmatism_angle=float(defocus_angle), dtype=torch.float64, **ctf_kwargs).to(dtype)
+        astigmatism_angle=float(defocus_angle), dtype=torch.float64, **kw).to(dtype)
     filt = projective_filter.to(dtype)[None, None] * ctf
     nb = math.ceil(euler_angle_offsets.shape[0] / batch_size)
     for i in range(nb):

@@ -1,0 +1,131 @@
+"""GPU parity of the HIP match-template search against the CPU oracle.
+
+Tolerances (float32 path):
+* MIP within 1e-4 of the float32 oracle, relative to max |MIP|; error against the
+  float64 oracle reported and bounded at the same level.
+* correlation mean / std (reference "variance" = std) within 1e-4 relative of the
+  float64 oracle (the HIP sums are compensated, the float32 oracle's are not).
+* best orientation / defocus identical at every pixel whose top-1 / top-2 margin
+  exceeds 1e-4 * max |MIP|.
+* z-score peak lists (zscore cutoff + greedy distance filter) identical.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from leopard_em_amd import core_match_template, synthetic
+from oracle import filters as of
+from oracle import geometry as og
+from oracle import peaks as op
+from oracle import search as osr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _inputs(H, W, h, D, C, O, seed, n_particles=3):
+    t = synthetic.make_blob_template(seed, h, 200, 0.35 * h)
+    vol = t.volume()
+    mg = synthetic.make_micrograph(t, seed, (H, W), n_particles, 12200.0, 11800.0, 30.0, snr=0.5)
+    img_dft, wf = of.prepare_image_dft(mg.image, (h, h // 2 + 1))
+    tdft = of.volume_to_rfft_fourier_slice(vol)
+    dfs = torch.linspace(-200.0, 200.0, D) if D > 1 else torch.tensor([0.0])
+    pix = torch.linspace(-0.02, 0.02, C) if C > 1 else torch.tensor([0.0])
+    ctf = of.calculate_ctf_filter_stack_full_args(
+        (h, h), 12200.0, 11800.0, dfs, pix, astigmatism_angle=30.0, voltage=300.0, spherical_aberration=2.7,
+        amplitude_contrast_ratio=0.07, phase_shift=0.0, pixel_size=1.0).float()
+    eul = torch.tensor(np.concatenate([mg.eulers, synthetic.random_uniform_rotations(
+        np.random.default_rng(seed + 10), O - n_particles)]), dtype=torch.float32)
+    return (img_dft.to(torch.complex64), tdft.to(torch.complex64), ctf, wf.float(), dfs, pix, eul)
+
+
+def _top2_margin(args):
+    img_dft, tdft, ctf, wf, dfs, pix, eul = args
+    filt = ctf.double() * wf.double()[None, None]
+    rot = og.euler_to_rotmat_zyz(eul, torch.float64)
+    cc = osr.batched_cross_correlate(img_dft.to(torch.complex128), tdft.to(torch.complex128), rot, filt)
+    H, W = cc.shape[-2:]
+    top = torch.topk(cc.reshape(-1, H, W), 2, dim=0).values
+    return top[0] - top[1]
+
+
+CASES = [
+    # H, W, h, D, C, O, B
+    (256, 256, 64, 3, 1, 16, 4),
+    (256, 512, 32, 2, 2, 12, 3),
+    (512, 512, 64, 1, 1, 24, 8),
+    (1024, 512, 128, 2, 1, 8, 4),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "H{}W{}h{}D{}C{}O{}B{}".format(*c))
+def test_match_parity_against_oracle(case, cuda_device):
+    H, W, h, D, C, O, B = case
+    print(f"case {case}", flush=True)
+    args = _inputs(H, W, h, D, C, O, seed=H + W + h)
+    got = core_match_template(*args, device=cuda_device, orientation_batch_size=B)
+    r32 = osr.core_match_template(*args, orientation_batch_size=B)
+    r64 = osr.core_match_template(*args, orientation_batch_size=B, dtype=torch.float64)
+    scale = float(r64["mip"].abs().max())
+    e32 = float((got["mip"].double() - r32["mip"].double()).abs().max()) / scale
+    e64 = float((got["mip"].double() - r64["mip"]).abs().max()) / scale
+    print(f"mip rel err vs fp32 {e32:.2e} vs fp64 {e64:.2e}", flush=True)
+    assert e32 < TOL and e64 < TOL
+    for k in ("correlation_mean", "correlation_variance"):
+        ref = r64[k]
+        err = float((got[k].double() - ref).abs().max() / ref.abs().max())
+        print(f"{k} rel err vs fp64 {err:.2e}", flush=True)
+        assert err < TOL
+    margin = _top2_margin(args)
+    ok = margin > TOL * scale
+    for k in ("best_phi", "best_theta", "best_psi", "best_defocus"):
+        agree = (got[k] == r32[k]) | ~ok
+        assert bool(agree.all()), f"{k}: {int((~agree).sum())} pixels differ"
+    assert got["total_projections"] == O * D * C
+    # z-score peak lists
+    cut = op.gaussian_noise_zscore_cutoff(H * W * O * D * C, 1.0)
+    cut = min(cut, float(r64["scaled_mip"].max()) - 0.5)
+    gy, gx = op.find_peaks_from_zscore(got["scaled_mip"], cut)
+    ry, rx = op.find_peaks_from_zscore(r64["scaled_mip"].float(), cut)
+    assert gy.tolist() == ry.tolist() and gx.tolist() == rx.tolist()
+
+
+def test_match_partial_last_batch_exact_index(cuda_device):
+    """O % B != 0 with D > 1: the HIP path decodes every correlation to its true index."""
+    args = _inputs(256, 256, 32, 3, 1, 10, seed=7)
+    got = core_match_template(*args, device=cuda_device, orientation_batch_size=4)
+    ref = osr.core_match_template(*args, orientation_batch_size=10)  # one full batch: no quirk
+    scale = float(ref["mip"].abs().max())
+    assert float((got["mip"] - ref["mip"]).abs().max()) / scale < TOL
+    margin = _top2_margin(args)
+    ok = margin > TOL * scale
+    assert bool(((got["best_defocus"] == ref["best_defocus"]) | ~ok).all())
+    assert bool(((got["best_phi"] == ref["best_phi"]) | ~ok).all())
+
+
+def test_match_recovers_embedded_particles(cuda_device):
+    """The true orientations are in the grid: each particle's peak sits at its top-left."""
+    H = W = 512
+    h = 64
+    t = synthetic.make_blob_template(3, h, 300, 22.0)
+    vol = t.volume()
+    mg = synthetic.make_micrograph(t, 3, (H, W), 4, 12200.0, 11800.0, 30.0, snr=0.2)
+    img_dft, wf = of.prepare_image_dft(mg.image, (h, h // 2 + 1))
+    tdft = of.volume_to_rfft_fourier_slice(vol)
+    dfs = torch.tensor([0.0])
+    ctf = of.calculate_ctf_filter_stack_full_args(
+        (h, h), 12200.0, 11800.0, dfs, torch.tensor([0.0]), astigmatism_angle=30.0, voltage=300.0,
+        spherical_aberration=2.7, amplitude_contrast_ratio=0.07, phase_shift=0.0, pixel_size=1.0).float()
+    eul = torch.tensor(np.concatenate([mg.eulers, synthetic.random_uniform_rotations(
+        np.random.default_rng(99), 60)]), dtype=torch.float32)
+    got = core_match_template(img_dft.to(torch.complex64), tdft.to(torch.complex64), ctf, wf.float(), dfs,
+                              torch.tensor([0.0]), eul, device=cuda_device, orientation_batch_size=8)
+    z = got["scaled_mip"]
+    for i, (Y, X) in enumerate(mg.centers_yx):
+        y, x = int(round(Y)) - h // 2, int(round(X)) - h // 2
+        win = z[y - 1:y + 2, x - 1:x + 2]
+        assert float(win.max()) > 5.0
+        iy, ix = np.unravel_index(int(win.argmax()), win.shape)
+        assert abs(float(got["best_phi"][y - 1 + iy, x - 1 + ix]) - float(eul[i, 0])) < 1e-3
