@@ -197,8 +197,12 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict_
 #pragma unroll
     for (int j = 0; j < R; ++j) a[j] = v[q + j * NB];
     if constexpr (NS > 1) {
+      // opaque copy of the table pointer: keeps the twiddle loads inside the caller's
+      // loops instead of hoisting them (and their registers) out of the loop
+      const float2* twp = tw;
+      asm volatile("" : "+s"(twp));
       float2 w[R];
-      twiddles<R, INV>(w, tw, k, TW_N / (NS * R));
+      twiddles<R, INV>(w, twp, k, TW_N / (NS * R));
 #pragma unroll
       for (int j = 1; j < R; ++j) a[j] = c_mul(a[j], w[j]);
       dft<R, INV>(a);

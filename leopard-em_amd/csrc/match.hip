@@ -404,15 +404,18 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
   float* lim = lre + LF;
   const int p = blockIdx.x;  // row-tile index
   const int y = p * TR + r;
+  // running max / sum / sum of squares per pixel; the correlation index of the max is
+  // kept as a 16-bit step-local index, two pixels per register (K < 65536)
   float m[2 * V], s[2 * V], q[2 * V];
-  int ix[2 * V];
+  unsigned ix2[V];
 #pragma unroll
   for (int j = 0; j < 2 * V; ++j) {
     m[j] = -INFINITY;
     s[j] = 0.f;
     q[j] = 0.f;
-    ix[j] = -1;
   }
+#pragma unroll
+  for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
   const int tile_elems = G * TR * CW;
   for (int k = 0; k < K; ++k) {
     const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
@@ -428,6 +431,8 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
     }
     __syncthreads();
     float2 v[V];
+    const float2* twp = tw;
+    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int mm = t + i * T;
@@ -438,24 +443,22 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
         A.y = 0.f;
         Bc.y = 0.f;
       }
-      const float2 w = c_conj(tw[mm * (TW_N / WS)]);
+      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
       const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
     __syncthreads();
     fft_wg<M, V, true>(v, lre, lim, t, tw);
-    const int bb = k % nb, dd = (k / nb) % nd, ci = k / (nb * nd);
-    const int gidx = ci * nd * O + dd * O + o0 + bb;
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
       if (a0 > m[2 * i]) {
         m[2 * i] = a0;
-        ix[2 * i] = gidx;
+        ix2[i] = (ix2[i] & 0xffff0000u) | (unsigned)k;
       }
       if (a1 > m[2 * i + 1]) {
         m[2 * i + 1] = a1;
-        ix[2 * i + 1] = gidx;
+        ix2[i] = (ix2[i] & 0x0000ffffu) | ((unsigned)k << 16);
       }
       s[2 * i] += a0;
       s[2 * i + 1] += a1;
@@ -473,8 +476,10 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
       const int j = 2 * i + h;
       const float mo = mip[e];
       if (m[j] > mo) {
+        const int kk = (int)((ix2[i] >> (16 * h)) & 0xffffu);
+        const int bb = kk % nb, dd = (kk / nb) % nd, ci = kk / (nb * nd);
         mip[e] = m[j];
-        bidx[e] = ix[j];
+        bidx[e] = ci * nd * O + dd * O + o0 + bb;
       }
       float sh = shi[e], sl = slo[e];
       two_sum(sh, sl, s[j]);
@@ -606,20 +611,13 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   float* lim = lre + LF;
   const int p = blockIdx.x;
   const int y = p * TR + r;
-  float mu[2 * V], sd[2 * V], zb[2 * V];
-  int kb[2 * V];
+  float zb[2 * V];
+  unsigned kb2[V];
   float ccmax = -INFINITY;
 #pragma unroll
-  for (int i = 0; i < V; ++i)
+  for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
-    for (int hh = 0; hh < 2; ++hh) {
-      const int x = 2 * (t + i * T) + hh;
-      const bool ok = y < crop_h && x < crop_w;
-      mu[2 * i + hh] = ok ? mean[(long)y * crop_w + x] : 0.f;
-      sd[2 * i + hh] = ok ? stdv[(long)y * crop_w + x] : 1.f;
-      zb[2 * i + hh] = -INFINITY;
-      kb[2 * i + hh] = -1;
-    }
+  for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
   const int tile_elems = G * TR * CW;
   for (int k = 0; k < K; ++k) {
     const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
@@ -635,6 +633,8 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
     }
     __syncthreads();
     float2 v[V];
+    const float2* twp = tw;
+    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int mm = t + i * T;
@@ -645,7 +645,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
         A.y = 0.f;
         Bc.y = 0.f;
       }
-      const float2 w = c_conj(tw[mm * (TW_N / WS)]);
+      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
       const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
@@ -659,10 +659,11 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
         if (y < crop_h && x < crop_w) {
           const float c = hh ? v[i].y : v[i].x;
           ccmax = fmaxf(ccmax, c);
-          const float z = (c - mu[2 * i + hh]) / sd[2 * i + hh];
+          const long e = (long)y * crop_w + x;
+          const float z = (c - mean[e]) / stdv[e];
           if (z > zb[2 * i + hh]) {
             zb[2 * i + hh] = z;
-            kb[2 * i + hh] = k;
+            kb2[i] = hh ? ((kb2[i] & 0x0000ffffu) | ((unsigned)k << 16)) : ((kb2[i] & 0xffff0000u) | (unsigned)k);
           }
         }
       }
@@ -673,12 +674,13 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
 #pragma unroll
     for (int hh = 0; hh < 2; ++hh) {
       const int x = 2 * (t + i * T) + hh;
-      if (y < crop_h && x < crop_w && kb[2 * i + hh] >= 0) {
+      const unsigned kk = (kb2[i] >> (16 * hh)) & 0xffffu;
+      if (y < crop_h && x < crop_w && kk != 0xffffu) {
         const int pix = y * crop_w + x;
         if (ref_better(zb[2 * i + hh], pix, b.z, b.pix)) {
           b.z = zb[2 * i + hh];
           b.pix = pix;
-          b.k = kb[2 * i + hh];
+          b.k = (int)kk;
         }
       }
     }
@@ -1049,6 +1051,7 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   if ((long)p->C * p->D * O > 2147483647L) return fail("search space exceeds int32 global index");
   hipStream_t st = (hipStream_t)stream;
   const int K = count * p->D * p->C;
+  if (K >= 65535) return fail("orientations x defocus x pixel sizes per step must be < 65535");
   const long ns = (long)count * p->h * p->wr;
   k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S);
   TDTM_LAUNCHED();
@@ -1091,6 +1094,7 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
       !ctf_params || !projective_filter || !corr_mean || !corr_std || !out)
     return fail("null argument");
   if (n_off < 1) return fail("no orientation offsets");
+  if ((long)p->B * p->D * p->C >= 65535) return fail("batch x defocus x pixel sizes must be < 65535");
   hipStream_t st = (hipStream_t)stream;
   const int crop_h = p->H - p->h + 1, crop_w = p->W - p->h + 1;
   const int nbatch = (n_off + p->B - 1) / p->B;
