@@ -84,8 +84,10 @@ def main():
     ap.add_argument("--batch", type=int, default=8, help="orientations per step")
     ap.add_argument("--size", type=int, default=4096, help="image edge (H = W)")
     ap.add_argument("--template", type=int, default=512)
-    ap.add_argument("--cpu-orientations", type=int, default=2, help="CPU baseline sample (x 13 defocus)")
+    ap.add_argument("--cpu-orientations", type=int, default=16, help="CPU baseline sample (x 13 defocus)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--inputs", choices=("synthetic", "random"), default="synthetic",
+                    help="random: skip micrograph synthesis (timing-equivalent; for profiler runs)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -100,9 +102,17 @@ def main():
     dev = torch.device("cuda:0")
     H = W = args.size
     h = args.template
-    print(f"[bench] building synthetic inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
-    n_blobs = 5000 if h == 512 else max(100, int(5000 * (h / 512) ** 3))
-    img_dft, tdft, ctf, wf, dfs, _ = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
+    print(f"[bench] building {args.inputs} inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
+    if args.inputs == "synthetic":
+        n_blobs = 5000 if h == 512 else max(100, int(5000 * (h / 512) ** 3))
+        img_dft, tdft, ctf, wf, dfs, _ = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
+    else:
+        gen = torch.Generator(device=dev).manual_seed(0)
+        img_dft = torch.randn((H, W // 2 + 1), dtype=torch.complex64, device=dev, generator=gen)
+        tdft = torch.randn((h, h, h // 2 + 1), dtype=torch.complex64, device=dev, generator=gen)
+        dfs = torch.arange(-1200.0, 1201.0, 200.0)
+        ctf = torch.rand((1, dfs.shape[0], h, h // 2 + 1), device=dev, generator=gen)
+        wf = torch.rand((h, h // 2 + 1), device=dev, generator=gen)
     D = dfs.shape[0]
     B = args.batch
     n_orient = (args.warmup + args.steps) * B
@@ -157,7 +167,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded Gaussian-blob template and micrograph; random orientations)",
+        "data": ("synthetic (seeded Gaussian-blob template and micrograph; random orientations)"
+                 if args.inputs == "synthetic" else "synthetic (random spectra; random orientations)"),
         "config": {
             "workload": f"plain 2DTM search, {H}x{W} micrograph x {h}^3 template, {D} defocus, "
                         f"{B} orientations per step",

@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "fft.hpp"
+#include "fft4.hpp"
 #include "tdtm.h"
 
 namespace tdtm {
@@ -293,7 +294,9 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
                                                         float2* __restrict__ Rb) {
   constexpr int M = WS / 2;
   constexpr int T = M / V;
-  constexpr int LF = LdsSize<M>::floats;
+  constexpr bool FOUR = UseFour<M>::value;
+  constexpr int MF = FOUR ? M : 512;
+  constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
   __shared__ float lds[2 * LF];
   float* lre = lds;
   float* lim = lds + LF;
@@ -313,10 +316,17 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
     }
     v[i] = z;
   }
-  fft_wg<M, V, false, NZJ>(v, lre, lim, t, tw);
+  if constexpr (FOUR) {
+    fs_nat2perm<MF, false, NZJ>(v, lre, lim, t, tw);
+    __syncthreads();  // other waves may still use their rows for the wave-local exchanges
+  } else {
+    fft_wg<M, V, false, NZJ>(v, lre, lim, t, tw);
+  }
 #pragma unroll
   for (int i = 0; i < V; ++i) {
-    const int p = lds_pad(t + i * T);
+    int m = t + i * T;
+    if constexpr (FOUR) m = fs_index<MF>(t, i);
+    const int p = lds_pad(m);
     lre[p] = v[i].x;
     lim[p] = v[i].y;
   }
@@ -339,23 +349,35 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
 
 // Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
 // conjugate against the image spectrum column, inverse FFT along y; tiled store.
+// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
+// conjugate against the image spectrum column, inverse FFT along y; tiled store.
+// Sizes >= 512 use the four-step transform (the product is formed in its permuted
+// order, the image column is held in registers in that order); smaller sizes Stockham.
 template <int HH, int NZJ>
 __global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                         int hs, int G, int K, int kchunk, int ystore,
                                                         const float2* __restrict__ tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
-  constexpr int LF = LdsSize<HH>::floats;
+  constexpr bool FOUR = UseFour<HH>::value;
+  constexpr int LF = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
   __shared__ float lds[CW * 2 * LF];
-  const int c = threadIdx.x % CW, t = threadIdx.x / CW;
+  // four-step: the waves of one column are contiguous; Stockham: columns interleaved
+  const int c = FOUR ? threadIdx.x / T : threadIdx.x % CW;
+  const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
   float* lre = lds + c * 2 * LF;
   float* lim = lre + LF;
   const int g = blockIdx.x;
   const int k0 = blockIdx.y * kchunk;
   const int k1 = min(K, k0 + kchunk);
   float2 xi[V];
+  bool dc = false;
 #pragma unroll
-  for (int i = 0; i < V; ++i) xi[i] = Xt[((long)g * HH + t + i * T) * CW + c];
-  const bool dc = (g == 0 && c == 0 && t == 0);
+  for (int i = 0; i < V; ++i) {
+    int ky = t + i * T;
+    if constexpr (FOUR) ky = fs_index<(FOUR ? HH : 512)>(t, i);
+    xi[i] = Xt[((long)g * HH + ky) * CW + c];
+  }
+  dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
   for (int k = k0; k < k1; ++k) {
     const float2* rin = Rb + ((long)k * G + g) * hs * CW;
     float2 v[V];
@@ -364,11 +386,19 @@ __global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restric
       const int yy = t + i * T;
       v[i] = (i < NZJ && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
     }
-    fft_wg<HH, V, false, NZJ>(v, lre, lim, t, tw);
+    if constexpr (FOUR) {
+      fs_nat2perm<(FOUR ? HH : 512), false, NZJ>(v, lre, lim, t, tw);
+    } else {
+      fft_wg<HH, V, false, NZJ>(v, lre, lim, t, tw);
+    }
 #pragma unroll
     for (int i = 0; i < V; ++i) v[i] = c_mulc(xi[i], v[i]);
     if (dc) v[0] = make_float2(0.f, 0.f);
-    fft_wg<HH, V, true>(v, lre, lim, t, tw);
+    if constexpr (FOUR) {
+      fs_perm2nat<(FOUR ? HH : 512), true>(v, lre, lim, t, tw);
+    } else {
+      fft_wg<HH, V, true>(v, lre, lim, t, tw);
+    }
     float2* gout = Gb + (long)k * HH * G * CW;
 #pragma unroll
     for (int i = 0; i < V; ++i) {
@@ -397,7 +427,9 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
                                                             float* __restrict__ qlo) {
   constexpr int M = WS / 2;
   constexpr int T = M / V;
-  constexpr int LF = LdsSize<M>::floats;
+  constexpr bool FOUR = UseFour<M>::value;
+  constexpr int MF = FOUR ? M : 512;
+  constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
   __shared__ float lds[TR * 2 * LF];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   float* lre = lds + r * 2 * LF;
@@ -435,7 +467,8 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
     asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      const int mm = t + i * T;
+      int mm = t + i * T;
+      if constexpr (FOUR) mm = fs_index<MF>(t, i);
       float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
       const int bm = M - mm;
       float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
@@ -448,7 +481,11 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
     __syncthreads();
-    fft_wg<M, V, true>(v, lre, lim, t, tw);
+    if constexpr (FOUR) {
+      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
+    } else {
+      fft_wg<M, V, true>(v, lre, lim, t, tw);
+    }
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -603,7 +640,9 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
                                                                    RefineBest* __restrict__ part) {
   constexpr int M = WS / 2;
   constexpr int T = M / V;
-  constexpr int LF = LdsSize<M>::floats;
+  constexpr bool FOUR = UseFour<M>::value;
+  constexpr int MF = FOUR ? M : 512;
+  constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
   __shared__ float lds[TR * 2 * LF];
   __shared__ RefineBest red[TR * T];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
@@ -637,7 +676,8 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
     asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
 #pragma unroll
     for (int i = 0; i < V; ++i) {
-      const int mm = t + i * T;
+      int mm = t + i * T;
+      if constexpr (FOUR) mm = fs_index<MF>(t, i);
       float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
       const int bm = M - mm;
       float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
@@ -650,7 +690,11 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
     __syncthreads();
-    fft_wg<M, V, true>(v, lre, lim, t, tw);
+    if constexpr (FOUR) {
+      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
+    } else {
+      fft_wg<M, V, true>(v, lre, lim, t, tw);
+    }
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll

@@ -52,11 +52,15 @@ def _top2_margin(args):
 
 
 CASES = [
-    # H, W, h, D, C, O, B
+    # H, W, h, D, C, O, B   (column FFT = H points, row c2r = W/2-point complex FFT;
+    #                        sizes >= 512 take the four-step path, smaller ones Stockham)
     (256, 256, 64, 3, 1, 16, 4),
     (256, 512, 32, 2, 2, 12, 3),
     (512, 512, 64, 1, 1, 24, 8),
     (1024, 512, 128, 2, 1, 8, 4),
+    (1024, 2048, 64, 2, 1, 4, 4),
+    (2048, 4096, 64, 2, 1, 4, 4),
+    (4096, 1024, 128, 1, 1, 4, 2),
 ]
 
 

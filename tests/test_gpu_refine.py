@@ -11,7 +11,7 @@ from oracle import search as osr
 pytestmark = pytest.mark.gpu
 
 CTF_KW = dict(voltage=300.0, spherical_aberration=2.7, amplitude_contrast_ratio=0.07, phase_shift=0.0,
-              pixel_size=1.0, ctf_B_factor=0.0, template_shape=(32, 32), even_zernikes=None, odd_zernikes=None)
+              pixel_size=1.0, ctf_B_factor=0.0, even_zernikes=None, odd_zernikes=None)
 
 
 def _particles(n, box, h, seed):
@@ -34,9 +34,9 @@ def _particles(n, box, h, seed):
     return (torch.stack(stack), tdft, torch.tensor(np.array(poses), dtype=torch.float32), mean, std)
 
 
-@pytest.mark.parametrize("batch", [4, 32])
-def test_refine_parity_against_oracle(batch, cuda_device):
-    n, box, h = 3, 128, 32
+@pytest.mark.parametrize("batch,box,h", [(4, 128, 32), (32, 128, 32), (8, 1024, 64)])
+def test_refine_parity_against_oracle(batch, box, h, cuda_device):
+    n = 3
     stack, tdft, poses, mean, std = _particles(n, box, h, seed=5)
     g = np.arange(-2.0, 2.01, 2.0)
     offs = torch.tensor([[a, b, c] for a in g for b in g for c in g], dtype=torch.float32)  # 27 offsets
