@@ -49,6 +49,15 @@ int tdtm_match_plan_destroy(tdtm_match_plan_t plan);
 int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
 
 /*
+ * Launch-shape options of a plan (no counterpart in the reference; the autotuner in
+ * leopard_em_amd.backend.autotune sets them).  Keys: "colpass_columns" (1 or 2 columns
+ * per column-pass block), "colpass_prefetch" (0/1), "colpass_chunks" (correlation chunks
+ * per column group, 0 = automatic), "rowpass_async" (0/1: asynchronous double-buffered
+ * tile staging).  Results do not depend on the options.
+ */
+int tdtm_match_plan_set_option(tdtm_match_plan_t plan, const char* key, int value);
+
+/*
  * Load the pre-processed image spectrum (re-tiled and scaled by 1/(H W) inside the
  * plan).  Replaces image_dft.to(device) at core_match_template.py:448.
  */

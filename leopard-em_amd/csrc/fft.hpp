@@ -155,6 +155,14 @@ __device__ __forceinline__ void twiddles(float2 (&w)[R], const float2* __restric
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
+// Workgroup barrier that orders LDS accesses only: outstanding global loads (e.g.
+// asynchronous global->LDS prefetches of the next correlation) stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <int N>
 struct LdsSize {
   static constexpr int floats = N + N / 16 + 1;  // per component (index N is valid)
@@ -223,13 +231,13 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict_
     }
   }
   if constexpr (!LAST) {
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int p = lds_pad(t + i * T);
       v[i] = make_float2(lre[p], lim[p]);
     }
-    __syncthreads();
+    lds_barrier();
   }
 }
 

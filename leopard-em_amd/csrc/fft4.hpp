@@ -89,7 +89,7 @@ __device__ __forceinline__ void fs_nat2perm(float2 (&v)[8], float* __restrict__ 
     re[k1 * RS + t] = v[k1].x;
     im[k1 * RS + t] = v[k1].y;
   }
-  __syncthreads();
+  lds_barrier();
   float2 b[8];
 #pragma unroll
   for (int u = 0; u < SUB; ++u)
@@ -201,10 +201,10 @@ __device__ __forceinline__ void fs_perm2nat(float2 (&v)[8], float* __restrict__ 
       re[a] = b[u * NW + j].x;
       im[a] = b[u * NW + j].y;
     }
-  __syncthreads();
+  lds_barrier();
 #pragma unroll
   for (int k1 = 0; k1 < 8; ++k1) v[k1] = make_float2(re[k1 * RS + t], im[k1 * RS + t]);
-  __syncthreads();
+  lds_barrier();
   // stage 1^T: twiddle W_N^{k1 t}, radix-8 over k1 -> i
   twiddle_mul<8, INV>(v, twp, t, TW_N / N);
   dft<8, INV>(v);

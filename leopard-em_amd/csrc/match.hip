@@ -353,20 +353,24 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
 // conjugate against the image spectrum column, inverse FFT along y; tiled store.
 // Sizes >= 512 use the four-step transform (the product is formed in its permuted
 // order, the image column is held in registers in that order); smaller sizes Stockham.
-template <int HH, int NZJ>
-__global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
-                                                        int hs, int G, int K, int kchunk, int ystore,
-                                                        const float2* __restrict__ tw, float2* __restrict__ Gb) {
+// CWB = columns per block (1 or 2 of the CW columns of a tile); PREF = register prefetch
+// of the next correlation's column samples.
+template <int HH, int NZJ, int CWB, bool PREF>
+__global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
+                                                         int hs, int G, int K, int kchunk, int ystore,
+                                                         const float2* __restrict__ tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr bool FOUR = UseFour<HH>::value;
   constexpr int LF = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
-  __shared__ float lds[CW * 2 * LF];
+  static_assert(CWB == CW || (CWB == 1 && FOUR), "one column per block needs the four-step path");
+  __shared__ float lds[CWB * 2 * LF];
   // four-step: the waves of one column are contiguous; Stockham: columns interleaved
-  const int c = FOUR ? threadIdx.x / T : threadIdx.x % CW;
+  const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
   const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
-  float* lre = lds + c * 2 * LF;
+  float* lre = lds + cl * 2 * LF;
   float* lim = lre + LF;
-  const int g = blockIdx.x;
+  const int g = CWB == CW ? (int)blockIdx.x : (int)blockIdx.x >> 1;
+  const int c = CWB == CW ? cl : (int)blockIdx.x & 1;
   const int k0 = blockIdx.y * kchunk;
   const int k1 = min(K, k0 + kchunk);
   float2 xi[V];
@@ -378,14 +382,24 @@ __global__ __launch_bounds__(CW*(HH / V)) void k_colpass(const float2* __restric
     xi[i] = Xt[((long)g * HH + ky) * CW + c];
   }
   dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
-  for (int k = k0; k < k1; ++k) {
+  // register prefetch of the next correlation's (non-zero) column samples
+  constexpr int NL = NZJ < V ? NZJ : V;
+  float2 pre[NL];
+  auto load_col = [&](int k) {
     const float2* rin = Rb + ((long)k * G + g) * hs * CW;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int yy = t + i * T;
+      pre[i] = (yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+    }
+  };
+  if (PREF && k0 < k1) load_col(k0);
+  for (int k = k0; k < k1; ++k) {
+    if (!PREF) load_col(k);
     float2 v[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const int yy = t + i * T;
-      v[i] = (i < NZJ && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
-    }
+    for (int i = 0; i < V; ++i) v[i] = i < NL ? pre[i] : make_float2(0.f, 0.f);
+    if (PREF && k + 1 < k1) load_col(k + 1);
     if constexpr (FOUR) {
       fs_nat2perm<(FOUR ? HH : 512), false, NZJ>(v, lre, lim, t, tw);
     } else {
@@ -416,26 +430,125 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
   lo += err;
 }
 
-// Row pass: c2r along x (WS points) for TR rows, fused statistics update over all K
-// correlations of the step, then one merge into the running maps.
+// Row-pass geometry for a W-point row (M = W/2 complex points).
 template <int WS>
+struct RowCfg {
+  static constexpr int M = WS / 2;
+  static constexpr int T = M / V;
+  static constexpr bool FOUR = UseFour<M>::value;
+  static constexpr int MF = FOUR ? M : 512;
+  static constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;  // per component per row
+  static constexpr int G = (M + 1 + CW - 1) / CW;                          // tile columns
+  static constexpr int TILE = G * TR * CW;                                 // complex per row-tile block
+  static constexpr int REGION = TR * 2 * LF;                               // floats, FFT work area
+  static constexpr int BUF = REGION > 2 * TILE ? REGION : 2 * TILE;        // floats per LDS buffer
+};
+
+// Asynchronous copy of one row-tile block (TILE complex, contiguous in HBM) into LDS with
+// global_load_lds_dwordx4: wave w copies 64 x 16-byte chunks per round to consecutive
+// LDS addresses, so LDS holds an exact copy of the HBM layout.  Completion is awaited
+// with s_waitcnt vmcnt before the consuming barrier.
+template <int WS>
+__device__ __forceinline__ void tile_block_async(const float2* __restrict__ src, float* dst) {
+  constexpr int NF4 = RowCfg<WS>::TILE / 2;
+  constexpr int BLOCK = TR * RowCfg<WS>::T;             // threads per block (the launch size)
+  constexpr int WL = BLOCK < 64 ? BLOCK : 64;           // active lanes per wave
+  constexpr int STEP = BLOCK;                            // chunks copied per round by the block
+  static_assert(BLOCK % WL == 0, "block must be whole waves or a single partial wave");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  for (int base = wave * WL; base < NF4; base += STEP) {
+    if (base + lane < NF4)
+      __builtin_amdgcn_global_load_lds((const void*)(s4 + base + lane),
+                                       (__attribute__((address_space(3))) void*)(d4 + base), 16, 0, 0);
+  }
+}
+
+// Correlation loop shared by the match and refine row passes.  For every correlation k:
+// wait for its tile block (prefetched into one of two LDS buffers during k-1), start the
+// prefetch of k+1 into the other buffer, build the half-length complex input of the
+// c2r from the staged tile (imaginary parts of DC / Nyquist dropped), run the inverse
+// transform in the same buffer, and hand the natural-order result to `epi(k, v)`:
+// thread (row r, t) holds v[i] = (x[2n], x[2n+1]) for n = t + i*T of row y = p*TR + r.
+// Synchronous variant of the tile copy (plain vector loads, then LDS stores).
+template <int WS>
+__device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, float* dst) {
+  constexpr int NF4 = RowCfg<WS>::TILE / 2;
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+  float4* d4 = reinterpret_cast<float4*>(dst);
+  for (int i = threadIdx.x; i < NF4; i += blockDim.x) d4[i] = s4[i];
+}
+
+template <int WS, bool ASYNC, class Epi>
+__device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p,
+                                             const float2* __restrict__ tw, float* lds, Epi&& epi) {
+  using C = RowCfg<WS>;
+  constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  const long kstride = (long)(HH / TR) * C::TILE;
+  const float2* gsrc = Gb + (long)p * C::TILE;
+  if (ASYNC && K > 0) tile_block_async<WS>(gsrc, lds);
+  for (int k = 0; k < K; ++k) {
+    float* cur = ASYNC ? lds + (k & 1) * C::BUF : lds;
+    float* nxt = lds + ((k + 1) & 1) * C::BUF;
+    if (ASYNC) {
+      __builtin_amdgcn_s_waitcnt(0);  // this thread's part of tile block k has landed
+      __syncthreads();                // everybody's part landed; iteration k-1 is done with nxt
+      if (k + 1 < K) tile_block_async<WS>(gsrc + (k + 1) * kstride, nxt);
+    } else {
+      tile_block_sync<WS>(gsrc + k * kstride, cur);
+      __syncthreads();
+    }
+    const float2* stg = reinterpret_cast<const float2*>(cur);
+    float2 v[V];
+    const float2* twp = tw;
+    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      int mm = t + i * T;
+      if constexpr (C::FOUR) mm = fs_index<MF>(t, i);
+      const int bm = M - mm;
+      float2 A = stg[((mm >> 1) * TR + r) * CW + (mm & 1)];
+      float2 Bc = stg[((bm >> 1) * TR + r) * CW + (bm & 1)];
+      Bc.y = -Bc.y;
+      if (mm == 0) {
+        A.y = 0.f;
+        Bc.y = 0.f;
+      }
+      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
+      const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
+      v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
+    }
+    lds_barrier();  // staged tile consumed: the buffer becomes the transform's work area
+    float* lre = cur + r * 2 * LF;
+    float* lim = lre + LF;
+    if constexpr (C::FOUR) {
+      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
+    } else {
+      fft_wg<M, V, true>(v, lre, lim, t, tw);
+      lds_barrier();
+    }
+    epi(k, v);
+  }
+}
+
+// Row pass of the search: fused MIP / argmax / sum / sum-of-squares over the K
+// correlations of the step, then one merge into the running maps.
+template <int WS, bool ASYNC>
 __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
                                                             int nb, int nd, int O, int o0,
                                                             const float2* __restrict__ tw, float* __restrict__ mip,
                                                             int* __restrict__ bidx, float* __restrict__ shi,
                                                             float* __restrict__ slo, float* __restrict__ qhi,
                                                             float* __restrict__ qlo) {
-  constexpr int M = WS / 2;
-  constexpr int T = M / V;
-  constexpr bool FOUR = UseFour<M>::value;
-  constexpr int MF = FOUR ? M : 512;
-  constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
-  __shared__ float lds[TR * 2 * LF];
+  using C = RowCfg<WS>;
+  constexpr int T = C::T;
+  __shared__ __attribute__((aligned(16))) float lds[(ASYNC ? 2 : 1) * C::BUF];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
-  float* lre = lds + r * 2 * LF;
-  float* lim = lre + LF;
   const int p = blockIdx.x;  // row-tile index
   const int y = p * TR + r;
+  (void)G;
   // running max / sum / sum of squares per pixel; the correlation index of the max is
   // kept as a 16-bit step-local index, two pixels per register (K < 65536)
   float m[2 * V], s[2 * V], q[2 * V];
@@ -448,44 +561,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
-  const int tile_elems = G * TR * CW;
-  for (int k = 0; k < K; ++k) {
-    const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
-    for (int e = threadIdx.x; e < tile_elems; e += blockDim.x) {
-      const int cc = e % CW, rr = (e / CW) % TR, gg = e / (CW * TR);
-      const int kx = gg * CW + cc;
-      if (kx <= M) {
-        const float2 val = gin[e];
-        const int pp = lds_pad(kx);
-        lds[rr * 2 * LF + pp] = val.x;
-        lds[rr * 2 * LF + LF + pp] = val.y;
-      }
-    }
-    __syncthreads();
-    float2 v[V];
-    const float2* twp = tw;
-    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
-#pragma unroll
-    for (int i = 0; i < V; ++i) {
-      int mm = t + i * T;
-      if constexpr (FOUR) mm = fs_index<MF>(t, i);
-      float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
-      const int bm = M - mm;
-      float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
-      if (mm == 0) {
-        A.y = 0.f;
-        Bc.y = 0.f;
-      }
-      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
-      const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
-      v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
-    }
-    __syncthreads();
-    if constexpr (FOUR) {
-      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
-    } else {
-      fft_wg<M, V, true>(v, lre, lim, t, tw);
-    }
+  rowpass_loop<WS, ASYNC>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -502,7 +578,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
       q[2 * i] = fmaf(a0, a0, q[2 * i]);
       q[2 * i + 1] = fmaf(a1, a1, q[2 * i + 1]);
     }
-  }
+  });
   const long rowbase = (long)y * WS;
 #pragma unroll
   for (int i = 0; i < V; ++i) {
@@ -638,18 +714,14 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
                                                                    const float* __restrict__ stdv,
                                                                    const float2* __restrict__ tw,
                                                                    RefineBest* __restrict__ part) {
-  constexpr int M = WS / 2;
-  constexpr int T = M / V;
-  constexpr bool FOUR = UseFour<M>::value;
-  constexpr int MF = FOUR ? M : 512;
-  constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
-  __shared__ float lds[TR * 2 * LF];
+  using C = RowCfg<WS>;
+  constexpr int T = C::T;
+  __shared__ __attribute__((aligned(16))) float lds[2 * C::BUF];
   __shared__ RefineBest red[TR * T];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
-  float* lre = lds + r * 2 * LF;
-  float* lim = lre + LF;
   const int p = blockIdx.x;
   const int y = p * TR + r;
+  (void)G;
   float zb[2 * V];
   unsigned kb2[V];
   float ccmax = -INFINITY;
@@ -657,44 +729,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  const int tile_elems = G * TR * CW;
-  for (int k = 0; k < K; ++k) {
-    const float2* gin = Gb + ((long)k * (HH / TR) + p) * tile_elems;
-    for (int e = threadIdx.x; e < tile_elems; e += blockDim.x) {
-      const int cc = e % CW, rr = (e / CW) % TR, gg = e / (CW * TR);
-      const int kx = gg * CW + cc;
-      if (kx <= M) {
-        const float2 val = gin[e];
-        const int pp = lds_pad(kx);
-        lds[rr * 2 * LF + pp] = val.x;
-        lds[rr * 2 * LF + LF + pp] = val.y;
-      }
-    }
-    __syncthreads();
-    float2 v[V];
-    const float2* twp = tw;
-    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
-#pragma unroll
-    for (int i = 0; i < V; ++i) {
-      int mm = t + i * T;
-      if constexpr (FOUR) mm = fs_index<MF>(t, i);
-      float2 A = make_float2(lre[lds_pad(mm)], lim[lds_pad(mm)]);
-      const int bm = M - mm;
-      float2 Bc = make_float2(lre[lds_pad(bm)], -lim[lds_pad(bm)]);
-      if (mm == 0) {
-        A.y = 0.f;
-        Bc.y = 0.f;
-      }
-      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
-      const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
-      v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
-    }
-    __syncthreads();
-    if constexpr (FOUR) {
-      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
-    } else {
-      fft_wg<M, V, true>(v, lre, lim, t, tw);
-    }
+  rowpass_loop<WS, true>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -711,7 +746,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
           }
         }
       }
-  }
+  });
   RefineBest b{-INFINITY, ccmax, 0x7fffffff, -1};
 #pragma unroll
   for (int i = 0; i < V; ++i)
@@ -733,14 +768,14 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int off = blockDim.x / 2; off > 0; off >>= 1) {
     if ((int)threadIdx.x < off) {
       RefineBest o = red[threadIdx.x + off];
-      RefineBest s = red[threadIdx.x];
-      s.cc = fmaxf(s.cc, o.cc);
-      if (ref_better(o.z, o.pix, s.z, s.pix)) {
-        s.z = o.z;
-        s.pix = o.pix;
-        s.k = o.k;
+      RefineBest sb = red[threadIdx.x];
+      sb.cc = fmaxf(sb.cc, o.cc);
+      if (ref_better(o.z, o.pix, sb.z, sb.pix)) {
+        sb.z = o.z;
+        sb.pix = o.pix;
+        sb.k = o.k;
       }
-      red[threadIdx.x] = s;
+      red[threadIdx.x] = sb;
     }
     __syncthreads();
   }
@@ -853,6 +888,11 @@ using namespace tdtm;
 
 struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
+  // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
+  int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
+  int opt_colpass_prefetch = 0;
+  int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
+  int opt_rowpass_async = 1;
   float2* tw = nullptr;
   float2* Xt = nullptr;
   float* filt = nullptr;
@@ -930,13 +970,24 @@ static void launch_rowfwd(tdtm_match_plan* p, int K, hipStream_t st) {
 
 template <int HH, int NZJ>
 static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t st) {
-  int nchunk = (2048 + p->G - 1) / p->G;
+  const int cols = (UseFour<HH>::value && p->opt_colpass_columns == 1) ? 1 : CW;
+  const int nblk_x = p->G * (CW / cols);
+  int nchunk = p->opt_colpass_chunks > 0 ? p->opt_colpass_chunks : (2048 + nblk_x - 1) / nblk_x;
   if (nchunk > K) nchunk = K;
   if (nchunk < 1) nchunk = 1;
   const int kchunk = (K + nchunk - 1) / nchunk;
   nchunk = (K + kchunk - 1) / kchunk;
-  dim3 grid(p->G, nchunk);
-  k_colpass<HH, NZJ><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
+  dim3 grid(nblk_x, nchunk);
+  const bool pf = p->opt_colpass_prefetch != 0;
+  if constexpr (UseFour<HH>::value) {
+    if (cols == 1) {
+      if (pf) k_colpass<HH, NZJ, 1, true><<<grid, HH / V, 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
+      else k_colpass<HH, NZJ, 1, false><<<grid, HH / V, 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
+      return;
+    }
+  }
+  if (pf) k_colpass<HH, NZJ, CW, true><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
+  else k_colpass<HH, NZJ, CW, false><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
 }
 
 template <int HH>
@@ -952,7 +1003,12 @@ template <int WS>
 static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                            float* slo, float* qhi, float* qlo, hipStream_t st) {
   dim3 grid(p->H / TR);
-  k_rowpass<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi,
+  if (!p->opt_rowpass_async) {
+    k_rowpass<WS, false><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx,
+                                                             shi, slo, qhi, qlo);
+    return;
+  }
+  k_rowpass<WS, true><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi,
                                                     slo, qhi, qlo);
 }
 
@@ -1044,6 +1100,25 @@ int tdtm_match_plan_destroy(tdtm_match_plan_t p) {
 }
 
 int64_t tdtm_match_plan_bytes(tdtm_match_plan_t p) { return p ? p->bytes : -1; }
+
+int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) {
+  if (!p || !key) return fail("null argument");
+  const std::string k(key);
+  if (k == "colpass_columns") {
+    if (value != 1 && value != 2) return fail("colpass_columns must be 1 or 2");
+    p->opt_colpass_columns = value;
+  } else if (k == "colpass_prefetch") {
+    p->opt_colpass_prefetch = value != 0;
+  } else if (k == "colpass_chunks") {
+    if (value < 0) return fail("colpass_chunks must be >= 0");
+    p->opt_colpass_chunks = value;
+  } else if (k == "rowpass_async") {
+    p->opt_rowpass_async = value != 0;
+  } else {
+    return fail("unknown option '" + k + "'");
+  }
+  return 0;
+}
 
 int tdtm_match_set_image(tdtm_match_plan_t p, const void* image_dft, void* stream) {
   if (!p || !image_dft) return fail("null argument");

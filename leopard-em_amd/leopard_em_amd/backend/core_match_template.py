@@ -60,6 +60,10 @@ class MatchPlan:
     def bytes(self) -> int:
         return int(self.lib.tdtm_match_plan_bytes(self.handle))
 
+    def set_option(self, key: str, value: int) -> None:
+        """Launch-shape option (see tdtm_match_plan_set_option); results are unaffected."""
+        _lib.check(self.lib.tdtm_match_plan_set_option(self.handle, key.encode(), int(value)))
+
     def _stream(self):
         return _lib.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 

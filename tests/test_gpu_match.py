@@ -148,3 +148,36 @@ def test_match_peak_list_identical(cuda_device):
     assert gy.tolist() == ry.tolist() and gx.tolist() == rx.tolist()
     for k in ("best_phi", "best_theta", "best_psi", "best_defocus"):
         assert got[k][gy, gx].tolist() == r64[k][ry, rx].float().tolist()
+
+
+@pytest.mark.parametrize("opts", [
+    {"colpass_columns": 1}, {"colpass_prefetch": 0}, {"rowpass_async": 0}, {"colpass_chunks": 3},
+], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
+def test_launch_options_do_not_change_results(opts, cuda_device):
+    from leopard_em_amd.backend import MatchPlan, raw_statistics
+
+    args = _inputs(1024, 1024, 64, 2, 1, 8, seed=21)
+    img_dft, tdft, ctf, wf, dfs, pix, eul = args
+    out = []
+    for o in ({}, opts):
+        plan = MatchPlan(1024, 1024, 64, 4, 2, 1, cuda_device)
+        for k, v in o.items():
+            plan.set_option(k, v)
+        plan.set_inputs(img_dft, ctf, wf)
+        plan.reset_stats()
+        plan.run(tdft.to(cuda_device), eul.to(cuda_device), 0, 8)
+        torch.cuda.synchronize()
+        out.append(raw_statistics(plan))
+    for k in out[0]:
+        assert torch.equal(out[0][k], out[1][k]), k
+
+
+def test_unknown_launch_option_raises(cuda_device):
+    from leopard_em_amd import _lib
+    from leopard_em_amd.backend import MatchPlan
+
+    plan = MatchPlan(256, 256, 64, 2, 1, 1, cuda_device)
+    with pytest.raises(_lib.TdtmError, match="unknown option"):
+        plan.set_option("no_such_option", 1)
+    with pytest.raises(_lib.TdtmError):
+        plan.set_option("colpass_columns", 3)
