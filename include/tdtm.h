@@ -51,7 +51,8 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
 /*
  * Launch-shape options of a plan (no counterpart in the reference; the autotuner in
  * leopard_em_amd.backend.autotune sets them).  Keys: "colpass_columns" (1 or 2 columns
- * per column-pass block), "colpass_prefetch" (0/1), "colpass_chunks" (correlation chunks
+ * per column-pass block), "colpass_pair" (0/1: two correlations per column-pass
+ * iteration sharing twiddles and barriers), "colpass_chunks" (correlation chunks
  * per column group, 0 = automatic), "rowpass_async" (0/1: asynchronous double-buffered
  * tile staging).  Results do not depend on the options.
  */

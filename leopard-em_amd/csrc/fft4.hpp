@@ -72,142 +72,238 @@ __device__ __forceinline__ void dft_any(float2 (&a)[R]) {
   if constexpr (R > 1) dft<R, INV>(a);
 }
 
-template <int N, bool INV, int NZJ>
-__device__ __forceinline__ void fs_nat2perm(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
-                                            const float2* __restrict__ tw) {
+// NT independent transforms at once (separate LDS regions re[n] / im[n]): twiddles are
+// generated once per stage for all of them and the workgroup barriers are shared.
+template <int R, bool INV, int NT>
+__device__ __forceinline__ void twiddle_mul_n(float2 (&a)[NT][R], const float2* __restrict__ tw, int k, int step) {
+  if constexpr (R > 1) {
+    float2 w[R];
+    twiddles<R, INV>(w, tw, k, step);
+#pragma unroll
+    for (int n = 0; n < NT; ++n)
+#pragma unroll
+      for (int j = 1; j < R; ++j) a[n][j] = c_mul(a[n][j], w[j]);
+  }
+}
+
+template <int N, bool INV, int NZJ, int NT>
+__device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
+                                              int t, const float2* __restrict__ tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
   const float2* twp = tw;
   asm volatile("" : "+s"(twp));
   // stage 1: radix-8 over i, twiddle W_N^{k1 t}
-  dft_sparse<8, NZJ, INV>(v, twp);
-  twiddle_mul<8, INV>(v, twp, t, TW_N / N);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft_sparse<8, NZJ, INV>(v[n], twp);
+  twiddle_mul_n<8, INV, NT>(v, twp, t, TW_N / N);
   // E1: row k1, column t
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) {
-    re[k1 * RS + t] = v[k1].x;
-    im[k1 * RS + t] = v[k1].y;
-  }
-  lds_barrier();
-  float2 b[8];
+  for (int n = 0; n < NT; ++n)
 #pragma unroll
-  for (int u = 0; u < SUB; ++u)
-#pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      const int a = (w * SUB + u) * RS + l + 64 * j;
-      b[u * NW + j] = make_float2(re[a], im[a]);
+    for (int k1 = 0; k1 < 8; ++k1) {
+      re[n][k1 * RS + t] = v[n][k1].x;
+      im[n][k1 * RS + t] = v[n][k1].y;
     }
+  lds_barrier();
+  float2 b[NT][8];
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int u = 0; u < SUB; ++u)
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const int a = (w * SUB + u) * RS + l + 64 * j;
+        b[n][u * NW + j] = make_float2(re[n][a], im[n][a]);
+      }
   // stage 2: radix-NW over j, twiddle W_S^{m1 l}
+  {
+    float2 wt[NW];
+    twiddles<NW, INV>(wt, twp, l, TW_N / F::S);
 #pragma unroll
-  for (int u = 0; u < SUB; ++u) {
-    float2 a[NW];
+    for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int j = 0; j < NW; ++j) a[j] = b[u * NW + j];
-    dft_any<NW, INV>(a);
-    twiddle_mul<NW, INV>(a, twp, l, TW_N / F::S);
+      for (int u = 0; u < SUB; ++u) {
+        float2 a[NW];
 #pragma unroll
-    for (int j = 0; j < NW; ++j) b[u * NW + j] = a[j];
+        for (int j = 0; j < NW; ++j) a[j] = b[n][u * NW + j];
+        dft_any<NW, INV>(a);
+#pragma unroll
+        for (int j = 1; j < NW; ++j) a[j] = c_mul(a[j], wt[j]);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) b[n][u * NW + j] = a[j];
+      }
   }
   // E2 (wave-local): group grp = u NW + m1 at row grp, column l
-  float* wr = re + w * 576;
-  float* wi = im + w * 576;
   wave_lds_order();
 #pragma unroll
-  for (int g = 0; g < 8; ++g) {
-    wr[g * 72 + l] = b[g].x;
-    wi[g * 72 + l] = b[g].y;
+  for (int n = 0; n < NT; ++n) {
+    float* wr = re[n] + w * 576;
+    float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+      wr[g * 72 + l] = b[n][g].x;
+      wi[g * 72 + l] = b[n][g].y;
+    }
   }
   wave_lds_order();
   const int grp = l >> 3, p = l & 7;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) b[q] = make_float2(wr[grp * 72 + p + 8 * q], wi[grp * 72 + p + 8 * q]);
+  for (int n = 0; n < NT; ++n) {
+    const float* wr = re[n] + w * 576;
+    const float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) b[n][q] = make_float2(wr[grp * 72 + p + 8 * q], wi[grp * 72 + p + 8 * q]);
+  }
   // stage 3: radix-8 over q, twiddle W_64^{p r}
-  dft<8, INV>(b);
-  twiddle_mul<8, INV>(b, twp, p, TW_N / 64);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft<8, INV>(b[n]);
+  twiddle_mul_n<8, INV, NT>(b, twp, p, TW_N / 64);
   // E3 (wave-local): [grp][r][p]
   wave_lds_order();
 #pragma unroll
-  for (int r = 0; r < 8; ++r) {
-    wr[grp * 72 + r * 9 + p] = b[r].x;
-    wi[grp * 72 + r * 9 + p] = b[r].y;
+  for (int n = 0; n < NT; ++n) {
+    float* wr = re[n] + w * 576;
+    float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      wr[grp * 72 + r * 9 + p] = b[n][r].x;
+      wi[grp * 72 + r * 9 + p] = b[n][r].y;
+    }
   }
   wave_lds_order();
   const int rr = l & 7;
 #pragma unroll
-  for (int pp = 0; pp < 8; ++pp) v[pp] = make_float2(wr[grp * 72 + rr * 9 + pp], wi[grp * 72 + rr * 9 + pp]);
+  for (int n = 0; n < NT; ++n) {
+    const float* wr = re[n] + w * 576;
+    const float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int pp = 0; pp < 8; ++pp)
+      v[n][pp] = make_float2(wr[grp * 72 + rr * 9 + pp], wi[grp * 72 + rr * 9 + pp]);
+  }
   // stage 4: radix-8 over p -> slot s
-  dft<8, INV>(v);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
 }
 
-// Transposed sequence: v[s] = Y[fs_index(t, s)] in, v[i] = y[t + T i] out.  Ends with a
-// workgroup barrier, so the LDS region is free again on return.
-template <int N, bool INV>
-__device__ __forceinline__ void fs_perm2nat(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
-                                            const float2* __restrict__ tw) {
+template <int N, bool INV, int NT>
+__device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
+                                              int t, const float2* __restrict__ tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
   const float2* twp = tw;
   asm volatile("" : "+s"(twp));
-  float* wr = re + w * 576;
-  float* wi = im + w * 576;
   const int grp = l >> 3, rr = l & 7;
   // stage 4^T: radix-8 over s -> p
-  dft<8, INV>(v);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
   // E3^T: lane (grp, r) holds p; lane (grp, p) needs r
   wave_lds_order();
 #pragma unroll
-  for (int pp = 0; pp < 8; ++pp) {
-    wr[grp * 72 + rr * 9 + pp] = v[pp].x;
-    wi[grp * 72 + rr * 9 + pp] = v[pp].y;
+  for (int n = 0; n < NT; ++n) {
+    float* wr = re[n] + w * 576;
+    float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int pp = 0; pp < 8; ++pp) {
+      wr[grp * 72 + rr * 9 + pp] = v[n][pp].x;
+      wi[grp * 72 + rr * 9 + pp] = v[n][pp].y;
+    }
   }
   wave_lds_order();
   const int p = l & 7;
-  float2 b[8];
+  float2 b[NT][8];
 #pragma unroll
-  for (int r = 0; r < 8; ++r) b[r] = make_float2(wr[grp * 72 + r * 9 + p], wi[grp * 72 + r * 9 + p]);
+  for (int n = 0; n < NT; ++n) {
+    const float* wr = re[n] + w * 576;
+    const float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) b[n][r] = make_float2(wr[grp * 72 + r * 9 + p], wi[grp * 72 + r * 9 + p]);
+  }
   // stage 3^T: twiddle W_64^{p r}, radix-8 over r -> q
-  twiddle_mul<8, INV>(b, twp, p, TW_N / 64);
-  dft<8, INV>(b);
+  twiddle_mul_n<8, INV, NT>(b, twp, p, TW_N / 64);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft<8, INV>(b[n]);
   // E2^T: lane (grp, p) holds column p + 8 q of row grp
   wave_lds_order();
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    wr[grp * 72 + p + 8 * q] = b[q].x;
-    wi[grp * 72 + p + 8 * q] = b[q].y;
+  for (int n = 0; n < NT; ++n) {
+    float* wr = re[n] + w * 576;
+    float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      wr[grp * 72 + p + 8 * q] = b[n][q].x;
+      wi[grp * 72 + p + 8 * q] = b[n][q].y;
+    }
   }
   wave_lds_order();
 #pragma unroll
-  for (int g = 0; g < 8; ++g) b[g] = make_float2(wr[g * 72 + l], wi[g * 72 + l]);
+  for (int n = 0; n < NT; ++n) {
+    const float* wr = re[n] + w * 576;
+    const float* wi = im[n] + w * 576;
+#pragma unroll
+    for (int g = 0; g < 8; ++g) b[n][g] = make_float2(wr[g * 72 + l], wi[g * 72 + l]);
+  }
   // stage 2^T: twiddle W_S^{m1 l}, radix-NW over m1 -> j
+  {
+    float2 wt[NW];
+    twiddles<NW, INV>(wt, twp, l, TW_N / F::S);
 #pragma unroll
-  for (int u = 0; u < SUB; ++u) {
-    float2 a[NW];
+    for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int j = 0; j < NW; ++j) a[j] = b[u * NW + j];
-    twiddle_mul<NW, INV>(a, twp, l, TW_N / F::S);
-    dft_any<NW, INV>(a);
+      for (int u = 0; u < SUB; ++u) {
+        float2 a[NW];
 #pragma unroll
-    for (int j = 0; j < NW; ++j) b[u * NW + j] = a[j];
+        for (int j = 0; j < NW; ++j) a[j] = b[n][u * NW + j];
+#pragma unroll
+        for (int j = 1; j < NW; ++j) a[j] = c_mul(a[j], wt[j]);
+        dft_any<NW, INV>(a);
+#pragma unroll
+        for (int j = 0; j < NW; ++j) b[n][u * NW + j] = a[j];
+      }
   }
   // E1^T: own rows, then all rows after the barrier
   wave_lds_order();
 #pragma unroll
-  for (int u = 0; u < SUB; ++u)
+  for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int j = 0; j < NW; ++j) {
-      const int a = (w * SUB + u) * RS + l + 64 * j;
-      re[a] = b[u * NW + j].x;
-      im[a] = b[u * NW + j].y;
-    }
+    for (int u = 0; u < SUB; ++u)
+#pragma unroll
+      for (int j = 0; j < NW; ++j) {
+        const int a = (w * SUB + u) * RS + l + 64 * j;
+        re[n][a] = b[n][u * NW + j].x;
+        im[n][a] = b[n][u * NW + j].y;
+      }
   lds_barrier();
 #pragma unroll
-  for (int k1 = 0; k1 < 8; ++k1) v[k1] = make_float2(re[k1 * RS + t], im[k1 * RS + t]);
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int k1 = 0; k1 < 8; ++k1) v[n][k1] = make_float2(re[n][k1 * RS + t], im[n][k1 * RS + t]);
   lds_barrier();
   // stage 1^T: twiddle W_N^{k1 t}, radix-8 over k1 -> i
-  twiddle_mul<8, INV>(v, twp, t, TW_N / N);
-  dft<8, INV>(v);
+  twiddle_mul_n<8, INV, NT>(v, twp, t, TW_N / N);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
+}
+
+// single-transform forms
+template <int N, bool INV, int NZJ>
+__device__ __forceinline__ void fs_nat2perm(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
+                                            const float2* __restrict__ tw) {
+  float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
+  float* const rr[1] = {re};
+  float* const ii[1] = {im};
+  fs_nat2perm_n<N, INV, NZJ, 1>(vv, rr, ii, t, tw);
+}
+
+template <int N, bool INV>
+__device__ __forceinline__ void fs_perm2nat(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
+                                            const float2* __restrict__ tw) {
+  float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
+  float* const rr[1] = {re};
+  float* const ii[1] = {im};
+  fs_perm2nat_n<N, INV, 1>(vv, rr, ii, t, tw);
 }
 
 }  // namespace tdtm

@@ -353,71 +353,110 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
 // conjugate against the image spectrum column, inverse FFT along y; tiled store.
 // Sizes >= 512 use the four-step transform (the product is formed in its permuted
 // order, the image column is held in registers in that order); smaller sizes Stockham.
-// CWB = columns per block (1 or 2 of the CW columns of a tile); PREF = register prefetch
-// of the next correlation's column samples.
-template <int HH, int NZJ, int CWB, bool PREF>
+// NT correlations of one column through the four-step transforms together (twiddles and
+// barriers shared): load the pruned samples, forward FFT, multiply by conj against the
+// image column (held in xi in the permuted order), inverse FFT, tiled store.
+template <int HH, int NZJ, int NT>
+__device__ __forceinline__ void colpass_four(const float2* __restrict__ Rb, int hs, int G, int g, int c, int t,
+                                             int k, int ystore, bool dc, const float2 (&xi)[V],
+                                             float* const (&re)[NT], float* const (&im)[NT],
+                                             const float2* __restrict__ tw, float2* __restrict__ Gb) {
+  constexpr int T = HH / V;
+  constexpr int NL = NZJ < V ? NZJ : V;
+  float2 v[NT][V];
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const float2* rin = Rb + ((long)(k + n) * G + g) * hs * CW;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int yy = t + i * T;
+      v[n][i] = (i < NL && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+    }
+  }
+  fs_nat2perm_n<HH, false, NZJ, NT>(v, re, im, t, tw);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) v[n][i] = c_mulc(xi[i], v[n][i]);
+    if (dc) v[n][0] = make_float2(0.f, 0.f);
+  }
+  fs_perm2nat_n<HH, true, NT>(v, re, im, t, tw);
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    float2* gout = Gb + (long)(k + n) * HH * G * CW;
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      const int yy = t + i * T;
+      if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[n][i];
+    }
+  }
+}
+
+// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
+// conjugate against the image spectrum column, inverse FFT along y; tiled store.
+// Sizes >= 512 use the four-step transform (the product is formed in its permuted
+// order, the image column is held in registers in that order) with NT correlations per
+// iteration; smaller sizes use the Stockham passes.  CWB = columns per block (1 or the
+// CW columns of a tile).
+template <int HH, int NZJ, int CWB, int NT>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int K, int kchunk, int ystore,
                                                          const float2* __restrict__ tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr bool FOUR = UseFour<HH>::value;
-  constexpr int LF = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
+  constexpr int HF = FOUR ? HH : 512;
+  constexpr int LF = FOUR ? FS<HF>::floats : LdsSize<HH>::floats;
   static_assert(CWB == CW || (CWB == 1 && FOUR), "one column per block needs the four-step path");
-  __shared__ float lds[CWB * 2 * LF];
+  static_assert(NT == 1 || FOUR, "paired correlations need the four-step path");
+  __shared__ float lds[CWB * NT * 2 * LF];
   // four-step: the waves of one column are contiguous; Stockham: columns interleaved
   const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
   const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
-  float* lre = lds + cl * 2 * LF;
-  float* lim = lre + LF;
   const int g = CWB == CW ? (int)blockIdx.x : (int)blockIdx.x >> 1;
   const int c = CWB == CW ? cl : (int)blockIdx.x & 1;
   const int k0 = blockIdx.y * kchunk;
   const int k1 = min(K, k0 + kchunk);
   float2 xi[V];
-  bool dc = false;
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     int ky = t + i * T;
-    if constexpr (FOUR) ky = fs_index<(FOUR ? HH : 512)>(t, i);
+    if constexpr (FOUR) ky = fs_index<HF>(t, i);
     xi[i] = Xt[((long)g * HH + ky) * CW + c];
   }
-  dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
-  // register prefetch of the next correlation's (non-zero) column samples
-  constexpr int NL = NZJ < V ? NZJ : V;
-  float2 pre[NL];
-  auto load_col = [&](int k) {
-    const float2* rin = Rb + ((long)k * G + g) * hs * CW;
-#pragma unroll
-    for (int i = 0; i < NL; ++i) {
-      const int yy = t + i * T;
-      pre[i] = (yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+  const bool dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
+  float* base = lds + cl * NT * 2 * LF;
+  if constexpr (FOUR) {
+    float* const re1[1] = {base};
+    float* const im1[1] = {base + LF};
+    int k = k0;
+    if constexpr (NT > 1) {
+      float* const re2[2] = {base, base + 2 * LF};
+      float* const im2[2] = {base + LF, base + 3 * LF};
+      for (; k + 1 < k1; k += 2) colpass_four<HF, NZJ, 2>(Rb, hs, G, g, c, t, k, ystore, dc, xi, re2, im2, tw, Gb);
     }
-  };
-  if (PREF && k0 < k1) load_col(k0);
-  for (int k = k0; k < k1; ++k) {
-    if (!PREF) load_col(k);
-    float2 v[V];
+    for (; k < k1; ++k) colpass_four<HF, NZJ, 1>(Rb, hs, G, g, c, t, k, ystore, dc, xi, re1, im1, tw, Gb);
+  } else {
+    float* lre = base;
+    float* lim = base + LF;
+    for (int k = k0; k < k1; ++k) {
+      const float2* rin = Rb + ((long)k * G + g) * hs * CW;
+      float2 v[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = i < NL ? pre[i] : make_float2(0.f, 0.f);
-    if (PREF && k + 1 < k1) load_col(k + 1);
-    if constexpr (FOUR) {
-      fs_nat2perm<(FOUR ? HH : 512), false, NZJ>(v, lre, lim, t, tw);
-    } else {
+      for (int i = 0; i < V; ++i) {
+        const int yy = t + i * T;
+        v[i] = (i < NZJ && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+      }
       fft_wg<HH, V, false, NZJ>(v, lre, lim, t, tw);
-    }
 #pragma unroll
-    for (int i = 0; i < V; ++i) v[i] = c_mulc(xi[i], v[i]);
-    if (dc) v[0] = make_float2(0.f, 0.f);
-    if constexpr (FOUR) {
-      fs_perm2nat<(FOUR ? HH : 512), true>(v, lre, lim, t, tw);
-    } else {
+      for (int i = 0; i < V; ++i) v[i] = c_mulc(xi[i], v[i]);
+      if (dc) v[0] = make_float2(0.f, 0.f);
       fft_wg<HH, V, true>(v, lre, lim, t, tw);
-    }
-    float2* gout = Gb + (long)k * HH * G * CW;
+      float2* gout = Gb + (long)k * HH * G * CW;
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const int yy = t + i * T;
-      if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+      for (int i = 0; i < V; ++i) {
+        const int yy = t + i * T;
+        if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+      }
     }
   }
 }
@@ -440,9 +479,14 @@ struct RowCfg {
   static constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;  // per component per row
   static constexpr int G = (M + 1 + CW - 1) / CW;                          // tile columns
   static constexpr int TILE = G * TR * CW;                                 // complex per row-tile block
-  static constexpr int REGION = TR * 2 * LF;                               // floats, FFT work area
+  static constexpr int RSTR = 2 * LF + 16;                                 // floats per row region
+  static constexpr int REGION = TR * RSTR;                                 // floats, FFT work area
   static constexpr int BUF = REGION > 2 * TILE ? REGION : 2 * TILE;        // floats per LDS buffer
 };
+
+// Padded index of staged row samples (split re / im per row): keeps the four-step read
+// order of the c2r input within 2-way bank conflicts.
+__device__ __forceinline__ int stg_pad(int i) { return i + (i >> 5); }
 
 // Asynchronous copy of one row-tile block (TILE complex, contiguous in HBM) into LDS with
 // global_load_lds_dwordx4: wave w copies 64 x 16-byte chunks per round to consecutive
@@ -471,13 +515,24 @@ __device__ __forceinline__ void tile_block_async(const float2* __restrict__ src,
 // c2r from the staged tile (imaginary parts of DC / Nyquist dropped), run the inverse
 // transform in the same buffer, and hand the natural-order result to `epi(k, v)`:
 // thread (row r, t) holds v[i] = (x[2n], x[2n+1]) for n = t + i*T of row y = p*TR + r.
-// Synchronous variant of the tile copy (plain vector loads, then LDS stores).
+// Synchronous tile staging: 16-byte loads (two columns of one row), stored de-interleaved
+// into the per-row split re / im regions at stg_pad positions.
 template <int WS>
 __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, float* dst) {
-  constexpr int NF4 = RowCfg<WS>::TILE / 2;
+  using C = RowCfg<WS>;
+  constexpr int NF4 = C::TILE / 2;
   const float4* s4 = reinterpret_cast<const float4*>(src);
-  float4* d4 = reinterpret_cast<float4*>(dst);
-  for (int i = threadIdx.x; i < NF4; i += blockDim.x) d4[i] = s4[i];
+  for (int f = threadIdx.x; f < NF4; f += blockDim.x) {
+    const float4 x = s4[f];
+    const int g = f / TR, rr = f % TR;
+    const int a = stg_pad(2 * g);
+    float* re = dst + rr * C::RSTR;
+    float* im = re + C::LF;
+    re[a] = x.x;
+    re[a + 1] = x.z;
+    im[a] = x.y;
+    im[a + 1] = x.w;
+  }
 }
 
 template <int WS, bool ASYNC, class Epi>
@@ -501,6 +556,8 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       __syncthreads();
     }
     const float2* stg = reinterpret_cast<const float2*>(cur);
+    const float* sre = cur + r * C::RSTR;
+    const float* sim = sre + LF;
     float2 v[V];
     const float2* twp = tw;
     asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
@@ -509,8 +566,14 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       int mm = t + i * T;
       if constexpr (C::FOUR) mm = fs_index<MF>(t, i);
       const int bm = M - mm;
-      float2 A = stg[((mm >> 1) * TR + r) * CW + (mm & 1)];
-      float2 Bc = stg[((bm >> 1) * TR + r) * CW + (bm & 1)];
+      float2 A, Bc;
+      if (ASYNC) {  // raw copy of the HBM tile layout
+        A = stg[((mm >> 1) * TR + r) * CW + (mm & 1)];
+        Bc = stg[((bm >> 1) * TR + r) * CW + (bm & 1)];
+      } else {
+        A = make_float2(sre[stg_pad(mm)], sim[stg_pad(mm)]);
+        Bc = make_float2(sre[stg_pad(bm)], sim[stg_pad(bm)]);
+      }
       Bc.y = -Bc.y;
       if (mm == 0) {
         A.y = 0.f;
@@ -521,7 +584,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
     lds_barrier();  // staged tile consumed: the buffer becomes the transform's work area
-    float* lre = cur + r * 2 * LF;
+    float* lre = cur + r * C::RSTR;
     float* lim = lre + LF;
     if constexpr (C::FOUR) {
       fs_perm2nat<MF, true>(v, lre, lim, t, tw);
@@ -716,7 +779,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
                                                                    RefineBest* __restrict__ part) {
   using C = RowCfg<WS>;
   constexpr int T = C::T;
-  __shared__ __attribute__((aligned(16))) float lds[2 * C::BUF];
+  __shared__ __attribute__((aligned(16))) float lds[C::BUF];
   __shared__ RefineBest red[TR * T];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int p = blockIdx.x;
@@ -729,7 +792,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  rowpass_loop<WS, true>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, false>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -890,9 +953,9 @@ struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
-  int opt_colpass_prefetch = 0;
+  int opt_colpass_pair = 1;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
-  int opt_rowpass_async = 1;
+  int opt_rowpass_async = 0;   // 1: raw asynchronous copy; 0: de-interleaving staging
   float2* tw = nullptr;
   float2* Xt = nullptr;
   float* filt = nullptr;
@@ -970,24 +1033,28 @@ static void launch_rowfwd(tdtm_match_plan* p, int K, hipStream_t st) {
 
 template <int HH, int NZJ>
 static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t st) {
-  const int cols = (UseFour<HH>::value && p->opt_colpass_columns == 1) ? 1 : CW;
+  constexpr bool FOUR = UseFour<HH>::value;
+  const int cols = (FOUR && p->opt_colpass_columns == 1) ? 1 : CW;
+  const int nt = (FOUR && p->opt_colpass_pair) ? 2 : 1;
   const int nblk_x = p->G * (CW / cols);
   int nchunk = p->opt_colpass_chunks > 0 ? p->opt_colpass_chunks : (2048 + nblk_x - 1) / nblk_x;
   if (nchunk > K) nchunk = K;
   if (nchunk < 1) nchunk = 1;
-  const int kchunk = (K + nchunk - 1) / nchunk;
+  int kchunk = (K + nchunk - 1) / nchunk;
+  if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
   nchunk = (K + kchunk - 1) / kchunk;
   dim3 grid(nblk_x, nchunk);
-  const bool pf = p->opt_colpass_prefetch != 0;
-  if constexpr (UseFour<HH>::value) {
-    if (cols == 1) {
-      if (pf) k_colpass<HH, NZJ, 1, true><<<grid, HH / V, 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
-      else k_colpass<HH, NZJ, 1, false><<<grid, HH / V, 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
-      return;
-    }
+#define TDTM_COLPASS(CWB_, NT_) \
+  k_colpass<HH, NZJ, CWB_, NT_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb)
+  if constexpr (FOUR) {
+    if (cols == 1 && nt == 2) TDTM_COLPASS(1, 2);
+    else if (cols == 1) TDTM_COLPASS(1, 1);
+    else if (nt == 2) TDTM_COLPASS(CW, 2);
+    else TDTM_COLPASS(CW, 1);
+  } else {
+    TDTM_COLPASS(CW, 1);
   }
-  if (pf) k_colpass<HH, NZJ, CW, true><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
-  else k_colpass<HH, NZJ, CW, false><<<grid, CW * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb);
+#undef TDTM_COLPASS
 }
 
 template <int HH>
@@ -1107,8 +1174,8 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
   if (k == "colpass_columns") {
     if (value != 1 && value != 2) return fail("colpass_columns must be 1 or 2");
     p->opt_colpass_columns = value;
-  } else if (k == "colpass_prefetch") {
-    p->opt_colpass_prefetch = value != 0;
+  } else if (k == "colpass_pair") {
+    p->opt_colpass_pair = value != 0;
   } else if (k == "colpass_chunks") {
     if (value < 0) return fail("colpass_chunks must be >= 0");
     p->opt_colpass_chunks = value;
