@@ -53,8 +53,9 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
  * leopard_em_amd.backend.autotune sets them).  Keys: "colpass_columns" (1 or 2 columns
  * per column-pass block), "colpass_pair" (0/1: two correlations per column-pass
  * iteration sharing twiddles and barriers), "colpass_chunks" (correlation chunks
- * per column group, 0 = automatic), "rowpass_async" (0/1: asynchronous double-buffered
- * tile staging).  Results do not depend on the options.
+ * per column group, 0 = automatic), "rowpass_staging" (0: synchronous de-interleaved,
+ * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat).
+ * Results do not depend on the options.
  */
 int tdtm_match_plan_set_option(tdtm_match_plan_t plan, const char* key, int value);
 

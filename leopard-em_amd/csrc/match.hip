@@ -521,8 +521,9 @@ template <int WS>
 __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, float* dst) {
   using C = RowCfg<WS>;
   constexpr int NF4 = C::TILE / 2;
+  constexpr int BLOCK = TR * C::T;
   const float4* s4 = reinterpret_cast<const float4*>(src);
-  for (int f = threadIdx.x; f < NF4; f += blockDim.x) {
+  for (int f = threadIdx.x; f < NF4; f += BLOCK) {
     const float4 x = s4[f];
     const int g = f / TR, rr = f % TR;
     const int a = stg_pad(2 * g);
@@ -535,22 +536,69 @@ __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, 
   }
 }
 
-template <int WS, bool ASYNC, class Epi>
+// Reformat of a raw staged tile block (HBM layout) into the de-interleaved per-row
+// re / im regions: consecutive 16-byte reads, paired 8-byte writes.
+template <int WS>
+__device__ __forceinline__ void tile_block_reformat(const float* raw, float* dst) {
+  using C = RowCfg<WS>;
+  constexpr int NF4 = C::TILE / 2;
+  constexpr int BLOCK = TR * C::T;
+  constexpr int ROUNDS = (NF4 + BLOCK - 1) / BLOCK;
+  const float4* s4 = reinterpret_cast<const float4*>(raw);
+#pragma unroll
+  for (int j = 0; j < ROUNDS; ++j) {
+    const int f = threadIdx.x + j * BLOCK;
+    if (f < NF4) {
+      const float4 x = s4[f];
+      const int g = f / TR, rr = f % TR;
+      const int a = stg_pad(2 * g);
+      float* re = dst + rr * C::RSTR;
+      float* im = re + C::LF;
+      re[a] = x.x;
+      re[a + 1] = x.z;
+      im[a] = x.y;
+      im[a + 1] = x.w;
+    }
+  }
+}
+
+// Tile staging modes of the row pass: 0 = synchronous load into the de-interleaved
+// per-row regions; 1 = asynchronous raw copy (double-buffered) read in place; 2 =
+// asynchronous raw copy of the next tile while the current one is reformatted into the
+// de-interleaved regions.
+template <int WS, int MODE>
+struct RowLds {
+  static constexpr int floats = MODE == 0 ? RowCfg<WS>::BUF
+                              : MODE == 1 ? 2 * RowCfg<WS>::BUF
+                                          : 2 * RowCfg<WS>::TILE + RowCfg<WS>::REGION;
+};
+
+template <int WS, int MODE, class Epi>
 __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p,
                                              const float2* __restrict__ tw, float* lds, Epi&& epi) {
   using C = RowCfg<WS>;
   constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
+  constexpr bool ASYNC = MODE == 1;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const long kstride = (long)(HH / TR) * C::TILE;
   const float2* gsrc = Gb + (long)p * C::TILE;
-  if (ASYNC && K > 0) tile_block_async<WS>(gsrc, lds);
+  float* raw2 = lds;                  // MODE 2: raw tile
+  float* work2 = lds + 2 * C::TILE;   // MODE 2: de-interleaved regions / transform work area
+  if (MODE == 1 && K > 0) tile_block_async<WS>(gsrc, lds);
+  if (MODE == 2 && K > 0) tile_block_async<WS>(gsrc, raw2);
   for (int k = 0; k < K; ++k) {
-    float* cur = ASYNC ? lds + (k & 1) * C::BUF : lds;
+    float* cur = MODE == 1 ? lds + (k & 1) * C::BUF : (MODE == 2 ? work2 : lds);
     float* nxt = lds + ((k + 1) & 1) * C::BUF;
-    if (ASYNC) {
+    if (MODE == 1) {
       __builtin_amdgcn_s_waitcnt(0);  // this thread's part of tile block k has landed
       __syncthreads();                // everybody's part landed; iteration k-1 is done with nxt
       if (k + 1 < K) tile_block_async<WS>(gsrc + (k + 1) * kstride, nxt);
+    } else if (MODE == 2) {
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();                // tile k landed in raw; iteration k-1 is done with work2
+      tile_block_reformat<WS>(raw2, work2);
+      __syncthreads();                // raw consumed, work2 staged
+      if (k + 1 < K) tile_block_async<WS>(gsrc + (k + 1) * kstride, raw2);
     } else {
       tile_block_sync<WS>(gsrc + k * kstride, cur);
       __syncthreads();
@@ -598,7 +646,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
 
 // Row pass of the search: fused MIP / argmax / sum / sum-of-squares over the K
 // correlations of the step, then one merge into the running maps.
-template <int WS, bool ASYNC>
+template <int WS, int MODE>
 __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
                                                             int nb, int nd, int O, int o0,
                                                             const float2* __restrict__ tw, float* __restrict__ mip,
@@ -607,7 +655,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
                                                             float* __restrict__ qlo) {
   using C = RowCfg<WS>;
   constexpr int T = C::T;
-  __shared__ __attribute__((aligned(16))) float lds[(ASYNC ? 2 : 1) * C::BUF];
+  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE>::floats];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int p = blockIdx.x;  // row-tile index
   const int y = p * TR + r;
@@ -624,7 +672,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
-  rowpass_loop<WS, ASYNC>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, MODE>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -792,7 +840,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  rowpass_loop<WS, false>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, 0>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -955,7 +1003,7 @@ struct tdtm_match_plan {
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
   int opt_colpass_pair = 1;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
-  int opt_rowpass_async = 0;   // 1: raw asynchronous copy; 0: de-interleaving staging
+  int opt_rowpass_staging = 0;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat
   float2* tw = nullptr;
   float2* Xt = nullptr;
   float* filt = nullptr;
@@ -1070,13 +1118,16 @@ template <int WS>
 static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                            float* slo, float* qhi, float* qlo, hipStream_t st) {
   dim3 grid(p->H / TR);
-  if (!p->opt_rowpass_async) {
-    k_rowpass<WS, false><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx,
-                                                             shi, slo, qhi, qlo);
-    return;
-  }
-  k_rowpass<WS, true><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi,
-                                                    slo, qhi, qlo);
+  const int threads = TR * (WS / 2) / V;
+  if (p->opt_rowpass_staging == 1)
+    k_rowpass<WS, 1><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
+                                               qlo);
+  else if (p->opt_rowpass_staging == 2)
+    k_rowpass<WS, 2><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
+                                               qlo);
+  else
+    k_rowpass<WS, 0><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
+                                               qlo);
 }
 
 template <int WS>
@@ -1179,8 +1230,9 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
   } else if (k == "colpass_chunks") {
     if (value < 0) return fail("colpass_chunks must be >= 0");
     p->opt_colpass_chunks = value;
-  } else if (k == "rowpass_async") {
-    p->opt_rowpass_async = value != 0;
+  } else if (k == "rowpass_staging") {
+    if (value < 0 || value > 2) return fail("rowpass_staging must be 0, 1 or 2");
+    p->opt_rowpass_staging = value;
   } else {
     return fail("unknown option '" + k + "'");
   }

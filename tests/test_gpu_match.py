@@ -151,7 +151,7 @@ def test_match_peak_list_identical(cuda_device):
 
 
 @pytest.mark.parametrize("opts", [
-    {"colpass_columns": 1}, {"colpass_pair": 0}, {"rowpass_async": 1}, {"colpass_chunks": 3},
+    {"colpass_columns": 1}, {"colpass_pair": 0}, {"rowpass_staging": 1}, {"rowpass_staging": 2}, {"colpass_chunks": 3},
     {"colpass_columns": 1, "colpass_pair": 0},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
