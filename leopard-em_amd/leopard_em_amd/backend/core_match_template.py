@@ -19,11 +19,18 @@ output dictionary.  Differences, all deliberate:
 
 from __future__ import annotations
 
+import json
+import os
+import time
 import warnings
+import zlib
 
+import numpy as np
 import torch
 
 from .. import _lib
+
+CHECKPOINT_FORMAT = "tdtm-match-stats-1"
 
 
 class MatchPlan:
@@ -97,6 +104,43 @@ class MatchPlan:
             self.H * self.W, _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums],
             *[_lib.ptr(t) for t in tens], self._stream()))
         self._keep = tens
+
+    def save_state(self, path: str, next_orientation: int, fingerprint: dict) -> None:
+        """Write the running statistics and the next orientation index to ``path``.
+
+        The maps are exact copies (mip, best index, compensated sums), so a search
+        resumed from the file reproduces an uninterrupted run bit for bit.  The file is
+        a numpy ``.npz`` written to a temporary name and renamed into place.
+        """
+        meta = dict(fingerprint, next_orientation=int(next_orientation), format=CHECKPOINT_FORMAT)
+        arrays = {"mip": self.mip, "best_index": self.best_index, "sum_hi": self.sums[0],
+                  "sum_lo": self.sums[1], "sq_hi": self.sums[2], "sq_lo": self.sums[3]}
+        host = {k: v.cpu().numpy() for k, v in arrays.items()}
+        host["meta"] = np.frombuffer(json.dumps(meta, sort_keys=True).encode(), dtype=np.uint8)
+        tmp = f"{path}.tmp{os.getpid()}.npz"
+        np.savez(tmp, **host)
+        os.replace(tmp, path)
+
+    def load_state(self, path: str, fingerprint: dict) -> int:
+        """Restore statistics saved by :meth:`save_state`; returns the next orientation.
+
+        Raises ValueError when the file belongs to a different search (shapes, grid,
+        defocus values or batch size differ).
+        """
+        with np.load(path, allow_pickle=False) as z:
+            meta = json.loads(bytes(z["meta"]).decode())
+            if meta.get("format") != CHECKPOINT_FORMAT:
+                raise ValueError(f"{path}: unknown checkpoint format {meta.get('format')!r}")
+            for k, v in fingerprint.items():
+                if meta.get(k) != v:
+                    raise ValueError(f"{path}: checkpoint is for a different search ({k}: {meta.get(k)!r} != {v!r})")
+            dst = [self.mip, self.best_index, *self.sums]
+            for name, t in zip(["mip", "best_index", "sum_hi", "sum_lo", "sq_hi", "sq_lo"], dst):
+                a = z[name]
+                if tuple(a.shape) != tuple(t.shape):
+                    raise ValueError(f"{path}: map {name} has shape {a.shape}, expected {tuple(t.shape)}")
+                t.copy_(torch.from_numpy(a))
+        return int(meta["next_orientation"])
 
     def finalize(self, euler_deg: torch.Tensor, defocus_values: torch.Tensor, total: int) -> dict:
         dev = self.device
@@ -183,11 +227,16 @@ def core_match_template(
         for other in plans[1:]:
             torch.cuda.synchronize(other.device)
             main.merge_from(other)
+    return _finish(main, defocus_values, O, D, C)
+
+
+def _finish(main: MatchPlan, defocus_values: torch.Tensor, O: int, D: int, C: int) -> dict:
+    with torch.cuda.device(main.device):
         dvals = defocus_values.to(main.device, torch.float32).contiguous()
         total = O * D * C
         maps = main.finalize(main._eul, dvals, total)
         torch.cuda.synchronize(main.device)
-    out = {
+    return {
         "mip": main.mip.cpu(),
         "scaled_mip": maps["scaled_mip"].cpu(),
         "best_phi": maps["best_phi"].cpu(),
@@ -200,7 +249,95 @@ def core_match_template(
         "total_orientations": O,
         "total_defocus": D,
     }
-    return out
+
+
+def _crc(t: torch.Tensor) -> int:
+    a = t.detach().contiguous().cpu()
+    if a.is_complex():
+        a = torch.view_as_real(a)
+    return zlib.crc32(a.numpy().tobytes())
+
+
+def search_fingerprint(image_dft, template_dft, ctf_filters, whitening_filter_template, defocus_values,
+                       pixel_values, euler_angles, orientation_batch_size) -> dict:
+    """Identity of a search for checkpoint matching: shapes, batch size and CRC-32s of
+    every input."""
+    return {
+        "image_shape": list(image_dft.shape),
+        "template_shape": list(template_dft.shape),
+        "num_orientations": int(euler_angles.shape[0]),
+        "num_defocus": int(defocus_values.shape[0]),
+        "num_pixel": int(pixel_values.shape[0]),
+        "orientation_batch_size": int(orientation_batch_size),
+        "crc": [_crc(t) for t in (image_dft, template_dft, ctf_filters, whitening_filter_template,
+                                  defocus_values, pixel_values, euler_angles)],
+    }
+
+
+# pylint: disable=too-many-arguments,too-many-locals
+def core_match_template_resumable(
+    image_dft: torch.Tensor,
+    template_dft: torch.Tensor,
+    ctf_filters: torch.Tensor,
+    whitening_filter_template: torch.Tensor,
+    defocus_values: torch.Tensor,
+    pixel_values: torch.Tensor,
+    euler_angles: torch.Tensor,
+    device: torch.device,
+    checkpoint_path: str,
+    orientation_batch_size: int = 1,
+    checkpoint_interval_s: float = 600.0,
+    max_batches: int | None = None,
+) -> dict[str, torch.Tensor] | None:
+    """``core_match_template`` on one device with checkpoint / resume of the statistics.
+
+    Implements the restart the reference lists as TODO
+    (``pydantic_models/results/match_template_result.py:51-56``).  If
+    ``checkpoint_path`` exists and belongs to the same search (same inputs and batch
+    size) the search continues from its next orientation; the statistics are written
+    every ``checkpoint_interval_s`` seconds and when the search stops.  With
+    ``max_batches`` the call stops after that many orientation batches and returns
+    None (the checkpoint then holds the progress); otherwise it returns the output
+    dictionary of ``core_match_template``.  A resumed search gives the same maps, bit
+    for bit, as an uninterrupted one.
+    """
+    devs = _as_device_list(device)
+    if len(devs) != 1:
+        raise ValueError("core_match_template_resumable runs on exactly one device")
+    dev = devs[0]
+    H = image_dft.shape[0]
+    W = 2 * (image_dft.shape[1] - 1)
+    h = template_dft.shape[1]
+    O, D, C = euler_angles.shape[0], defocus_values.shape[0], pixel_values.shape[0]
+    batch = max(1, min(orientation_batch_size, O))
+    fp = search_fingerprint(image_dft, template_dft, ctf_filters, whitening_filter_template, defocus_values,
+                            pixel_values, euler_angles, batch)
+    plan = MatchPlan(H, W, h, batch, D, C, dev)
+    with torch.cuda.device(dev):
+        plan.set_inputs(image_dft, ctf_filters, whitening_filter_template)
+        tdft = template_dft.to(dev, torch.complex64).contiguous()
+        eul = euler_angles.to(dev, torch.float32).contiguous()
+        plan._tdft, plan._eul = tdft, eul
+        plan.reset_stats()
+        start = 0
+        if os.path.exists(checkpoint_path):
+            start = plan.load_state(checkpoint_path, fp)
+        last = time.monotonic()
+        done = 0
+        i = start
+        while i < O:
+            if max_batches is not None and done >= max_batches:
+                break
+            plan.step(tdft, eul, i, min(batch, O - i))
+            i = min(O, i + batch)
+            done += 1
+            if time.monotonic() - last >= checkpoint_interval_s:
+                plan.save_state(checkpoint_path, i, fp)
+                last = time.monotonic()
+        plan.save_state(checkpoint_path, i, fp)
+        if i < O:
+            return None
+    return _finish(plan, defocus_values, O, D, C)
 
 
 def raw_statistics(plan: MatchPlan) -> dict[str, torch.Tensor]:
