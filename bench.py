@@ -24,7 +24,9 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "leopard-em_amd"))
 
 HBM_PEAK_GBS = 8000.0
-FULL_GRID_ORIENTATIONS = 1_584_480  # 2.5 deg / 1.5 deg uniform grid (docs/benchmarks.md:20 ~1.58M)
+# Published H100 80 GB rate of the reference on the same workload (docs/benchmarks.md:32).
+REFERENCE_H100_CORR_PER_S = 1650.8
+TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "match_step_4096x4096_t512_b8_d13_traffic.md")
 
 
 def algorithmic_bytes_per_correlation(H: int, W: int) -> int:
@@ -76,6 +78,18 @@ def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
     return n / dt, cores, n, dt
 
 
+def read_traffic(path: str):
+    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+    try:
+        with open(path, encoding="utf-8") as f:
+            for ln in f:
+                if ln.startswith("hbm_bytes_per_step:"):
+                    return int(ln.split(":", 1)[1].strip())
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +102,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--inputs", choices=("synthetic", "random"), default="synthetic",
                     help="random: skip micrograph synthesis (timing-equivalent; for profiler runs)")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run one device copy of this many MiB first (reference kernel for HBM byte counters)")
+    ap.add_argument("--no-phase-timing", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -98,6 +115,7 @@ def main():
     import torch
 
     from leopard_em_amd.backend import MatchPlan
+    from leopard_em_amd.utils.search import get_uniform_euler_angles
 
     dev = torch.device("cuda:0")
     H = W = args.size
@@ -115,9 +133,22 @@ def main():
         wf = torch.rand((h, h // 2 + 1), device=dev, generator=gen)
     D = dfs.shape[0]
     B = args.batch
-    n_orient = (args.warmup + args.steps) * B
-    g = torch.Generator().manual_seed(1)
-    eul = (torch.rand((max(n_orient, 4096), 3), generator=g) * torch.tensor([360.0, 180.0, 360.0])).to(dev)
+    n_orient = (args.warmup + 2 * args.steps) * B
+    # a strided slice of the production grid (out-of-plane 2.5 deg, in-plane 1.5 deg)
+    grid = get_uniform_euler_angles(psi_step=1.5, theta_step=2.5)
+    full_grid = grid.shape[0]
+    stride = max(1, full_grid // max(n_orient, 4096))
+    eul = grid[::stride][:max(n_orient, 4096)].contiguous().to(dev)
+
+    if args.copy_calibration_mib:
+        nbytes = args.copy_calibration_mib << 20
+        a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize(dev)
+        b.copy_(a)
+        torch.cuda.synchronize(dev)
+        print(f"[bench] copy calibration: {nbytes} bytes read + written", file=sys.stderr, flush=True)
+        del a, b
 
     plan = MatchPlan(H, W, h, B, D, 1, dev)
     plan.set_inputs(img_dft, ctf, wf)
@@ -147,6 +178,20 @@ def main():
     print(f"[bench] {args.steps} steps: {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms), {value:.1f} corr/s",
           file=sys.stderr, flush=True)
 
+    phases = None
+    if not args.no_phase_timing:
+        # same number of steps again, with HIP events between the phases of each step
+        plan.set_option("phase_timing", 1)
+        for i in range(args.steps):
+            plan.step(tdft, eul, (args.warmup + args.steps + i) * B, B)
+        ph, n = plan.phase_times()
+        plan.set_option("phase_timing", 0)
+        phases = {k: v / n for k, v in ph.items()}
+        print("[bench] phase ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()),
+              file=sys.stderr, flush=True)
+
+    traffic = read_traffic(TRAFFIC_SUMMARY) if (H, W, h, B, D) == (4096, 4096, 512, 8, 13) else None
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (float32 oracle) ...", file=sys.stderr, flush=True)
@@ -165,7 +210,9 @@ def main():
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": value / REFERENCE_H100_CORR_PER_S,
+        "baseline": {"value": REFERENCE_H100_CORR_PER_S, "hardware": "H100 80 GB",
+                     "source": "reference docs/benchmarks.md:32"},
         "dtype": "f32",
         "data": ("synthetic (seeded Gaussian-blob template and micrograph; random orientations)"
                  if args.inputs == "synthetic" else "synthetic (random spectra; random orientations)"),
@@ -173,16 +220,20 @@ def main():
             "workload": f"plain 2DTM search, {H}x{W} micrograph x {h}^3 template, {D} defocus, "
                         f"{B} orientations per step",
             "image": [H, W], "template": h, "defocus": D, "orientation_batch": B,
-            "full_grid_hours_extrapolated": FULL_GRID_ORIENTATIONS * D / value / 3600.0,
+            "orientations": "strided slice of the 2.5/1.5 deg uniform grid",
+            "full_grid_orientations": full_grid,
+            "full_grid_hours_extrapolated": full_grid * D / value / 3600.0,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "tdtm_match_step (fused slice..statistics launch sequence)",
+            "kernel": "tdtm_match_step launch sequence (k_slice, k_proj_*, k_rowfwd, k_colpass, k_rowpass)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "algorithmic_bytes_per_step": ab * B * D,
+            "phases_ms_per_step": phases,
         },
         "cpu_baseline": cpu,
     }

@@ -28,6 +28,9 @@ extern "C" {
 
 typedef struct tdtm_match_plan* tdtm_match_plan_t;
 
+/* Phases of a search step reported by tdtm_match_plan_phase_times. */
+#define TDTM_NUM_PHASES 4 /* project (slice + filter + irfft2 + normalise) | rowfwd | colpass | rowpass */
+
 /* Version of the ABI (bumped on any signature change). */
 int tdtm_abi_version(void);
 
@@ -52,12 +55,25 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
  * Launch-shape options of a plan (no counterpart in the reference; the autotuner in
  * leopard_em_amd.backend.autotune sets them).  Keys: "colpass_columns" (1 or 2 columns
  * per column-pass block), "colpass_pair" (0/1: two correlations per column-pass
- * iteration sharing twiddles and barriers), "colpass_chunks" (correlation chunks
- * per column group, 0 = automatic), "rowpass_staging" (0: synchronous de-interleaved,
- * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat).
+ * iteration sharing twiddles and barriers; default 0), "colpass_chunks" (correlation chunks
+ * per column group, 0 = automatic), "colpass_twiddles" (0: global table, 1: LDS copy
+ * for the four-step column transforms where it fits, default), "rowpass_staging" (0: synchronous de-interleaved,
+ * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat,
+ * 3: synchronous de-interleaved with batched loads; default 3), "rowpass_twiddles" (0: global
+ * table, 1: LDS copy, default; needs staging 0 or 3), "phase_timing" (0/1: record HIP events
+ * between the phases of every step).
  * Results do not depend on the options.
  */
 int tdtm_match_plan_set_option(tdtm_match_plan_t plan, const char* key, int value);
+
+/*
+ * With "phase_timing" on: waits for the recorded steps, adds up each phase's device
+ * time over them into out_ms[TDTM_NUM_PHASES] (milliseconds), stores the number of
+ * steps in *out_steps and releases the events.  No counterpart in the reference (its
+ * benchmark times whole calls, benchmark/benchmark_match_template.py:88-123); used by
+ * the benchmark and the launch-shape autotuner.
+ */
+int tdtm_match_plan_phase_times(tdtm_match_plan_t plan, double* out_ms, int* out_steps);
 
 /*
  * Load the pre-processed image spectrum (re-tiled and scaled by 1/(H W) inside the

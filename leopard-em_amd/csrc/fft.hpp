@@ -22,12 +22,15 @@ constexpr int TW_N = 8192;  // twiddle table length (largest supported row lengt
 
 __device__ __forceinline__ float2 c_add(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 c_sub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Products are written with explicit fused multiply-adds and the library is built with
+// -ffp-contract=off: every code shape (launch options, paired transforms) then rounds
+// identically, so results do not depend on the launch shape.
 __device__ __forceinline__ float2 c_mul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(fmaf(a.x, b.x, -(a.y * b.y)), fmaf(a.x, b.y, a.y * b.x));
 }
 // a * conj(b)
 __device__ __forceinline__ float2 c_mulc(float2 a, float2 b) {
-  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -(a.x * b.y)));
 }
 __device__ __forceinline__ float2 c_conj(float2 a) { return make_float2(a.x, -a.y); }
 __device__ __forceinline__ float2 c_scale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
@@ -72,13 +75,20 @@ template <bool INV>
 __device__ __forceinline__ void dft8(float2 (&a)[8]) {
   dft4<INV>(a[0], a[2], a[4], a[6]);
   dft4<INV>(a[1], a[3], a[5], a[7]);
-  float2 o1 = c_w8<INV>(a[3]);
+  // W8^1 a3 = r u1 and W8^3 a7 = r u3 with the unscaled u formed first; y = base +- r u
+  // as explicit fused multiply-adds
+  const float r = 0.70710678118654752440f;
+  const float2 u1 = INV ? make_float2(a[3].x - a[3].y, a[3].x + a[3].y)
+                        : make_float2(a[3].x + a[3].y, a[3].y - a[3].x);
+  const float2 u3 = INV ? make_float2(-(a[7].x + a[7].y), a[7].x - a[7].y)
+                        : make_float2(a[7].y - a[7].x, -(a[7].x + a[7].y));
   float2 o2 = c_rot<INV>(a[5]);
-  float2 o3 = c_w83<INV>(a[7]);
   float2 y0 = c_add(a[0], a[1]), y4 = c_sub(a[0], a[1]);
-  float2 y1 = c_add(a[2], o1), y5 = c_sub(a[2], o1);
+  float2 y1 = make_float2(fmaf(u1.x, r, a[2].x), fmaf(u1.y, r, a[2].y));
+  float2 y5 = make_float2(fmaf(-u1.x, r, a[2].x), fmaf(-u1.y, r, a[2].y));
   float2 y2 = c_add(a[4], o2), y6 = c_sub(a[4], o2);
-  float2 y3 = c_add(a[6], o3), y7 = c_sub(a[6], o3);
+  float2 y3 = make_float2(fmaf(u3.x, r, a[6].x), fmaf(u3.y, r, a[6].y));
+  float2 y7 = make_float2(fmaf(-u3.x, r, a[6].x), fmaf(-u3.y, r, a[6].y));
   a[0] = y0; a[1] = y1; a[2] = y2; a[3] = y3;
   a[4] = y4; a[5] = y5; a[6] = y6; a[7] = y7;
 }
@@ -128,8 +138,8 @@ __device__ __forceinline__ void dft(float2 (&a)[R]) {
 }
 
 // twiddle w^j (j = 1..R-1) for w = W_L^k, L = TW_N / step; products of <= 3 table entries
-template <int R, bool INV>
-__device__ __forceinline__ void twiddles(float2 (&w)[R], const float2* __restrict__ tw, int k, int step) {
+template <int R, bool INV, class TWT>
+__device__ __forceinline__ void twiddles(float2 (&w)[R], TWT tw, int k, int step) {
   auto ld = [&](int m) {
     float2 v = tw[m];
     return INV ? c_conj(v) : v;
@@ -155,6 +165,38 @@ __device__ __forceinline__ void twiddles(float2 (&w)[R], const float2* __restric
 
 __device__ __forceinline__ int lds_pad(int i) { return i + (i >> 4); }
 
+// p + 0 through an empty asm: the compiler cannot hoist loads through the result out of
+// the caller's loop (which would keep their values live in registers), while the
+// pointer keeps its provenance (a kernel argument, i.e. global memory), so the loads are
+// global_load and not flat_load (a flat load counts against lgkmcnt and would make every
+// LDS wait also wait for it).
+template <class T>
+__device__ __forceinline__ const T* opaque_offset(const T* p) {
+  int off = 0;
+  asm volatile("" : "+s"(off));
+  return p + off;
+}
+
+// Twiddle table in LDS: p[j] = tw[j << SH], for transforms whose table indices are all
+// multiples of 1 << SH (a W-point row pass uses multiples of TW_N / W).
+// log2(TW_N / n) for a transform length n that divides TW_N
+constexpr int tw_shift(int n) {
+  int s = 0;
+  while ((n << s) < TW_N) ++s;
+  return s;
+}
+
+template <int SH>
+struct TwLds {
+  const float2* p;
+  __device__ __forceinline__ float2 operator[](int m) const { return p[m >> SH]; }
+};
+
+template <int SH>
+__device__ __forceinline__ TwLds<SH> opaque_offset(TwLds<SH> t) {
+  return TwLds<SH>{opaque_offset(t.p)};
+}
+
 // Workgroup barrier that orders LDS accesses only: outstanding global loads (e.g.
 // asynchronous global->LDS prefetches of the next correlation) stay in flight.
 __device__ __forceinline__ void lds_barrier() {
@@ -170,29 +212,34 @@ struct LdsSize {
 
 // out[m] = sum_{j < NZJ} a[j] W_R^{jm}: first pass when only the leading NZJ inputs of
 // every butterfly can be non-zero (zero-padded transforms)
-template <int R, int NZJ, bool INV>
-__device__ __forceinline__ void dft_sparse(float2 (&a)[R], const float2* __restrict__ tw) {
+template <int R, int NZJ, bool INV, class TWT>
+__device__ __forceinline__ void dft_sparse(float2 (&a)[R], TWT tw) {
   if constexpr (NZJ >= R || NZJ > 2) {
     dft<R, INV>(a);
   } else if constexpr (NZJ == 1) {
 #pragma unroll
     for (int m = 1; m < R; ++m) a[m] = a[0];
   } else {
+    // a[m] = a0 + a1 W_R^m with the R-th roots of unity (R <= 8) as exact constants
+    static_assert(R <= 8, "two-input sparse butterflies up to radix 8");
+    (void)tw;
     const float2 a0 = a[0], a1 = a[1];
-    constexpr int step = TW_N / R;
+    constexpr float H = 0.70710678118654752f;
+    constexpr float WR[8] = {1.f, H, 0.f, -H, -1.f, -H, 0.f, H};   // cos(2 pi m / 8)
+    constexpr float WI[8] = {0.f, -H, -1.f, -H, 0.f, H, 1.f, H};   // -sin(2 pi m / 8)
 #pragma unroll
     for (int m = 0; m < R; ++m) {
-      float2 w = tw[m * step];
-      if (INV) w = c_conj(w);
+      const int e = m * (8 / R);
+      const float2 w = make_float2(WR[e], INV ? -WI[e] : WI[e]);
       a[m] = c_add(a0, c_mul(a1, w));
     }
   }
 }
 
 // One Stockham pass of a size-N transform (span NS -> NS*R).  v holds x[t + i*T].
-template <int N, int V, int NS, bool INV, int NZJ>
+template <int N, int V, int NS, bool INV, int NZJ, class TWT>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict__ lre, float* __restrict__ lim,
-                                              int t, const float2* __restrict__ tw) {
+                                              int t, TWT tw) {
   constexpr int T = N / V;
   constexpr int R = (N / NS < V) ? N / NS : V;
   constexpr int NB = V / R;
@@ -207,8 +254,7 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict_
     if constexpr (NS > 1) {
       // opaque copy of the table pointer: keeps the twiddle loads inside the caller's
       // loops instead of hoisting them (and their registers) out of the loop
-      const float2* twp = tw;
-      asm volatile("" : "+s"(twp));
+      const auto twp = opaque_offset(tw);
       float2 w[R];
       twiddles<R, INV>(w, twp, k, TW_N / (NS * R));
 #pragma unroll
@@ -242,9 +288,9 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict_
 }
 
 // Full transform; NZJ describes the zero padding of the input (first pass only).
-template <int N, int V, bool INV, int NZJ = 16, int NS = 1>
+template <int N, int V, bool INV, int NZJ = 16, int NS = 1, class TWT>
 __device__ __forceinline__ void fft_wg(float2 (&v)[V], float* __restrict__ lre, float* __restrict__ lim, int t,
-                                       const float2* __restrict__ tw) {
+                                       TWT tw) {
   constexpr int R = (N / NS < V) ? N / NS : V;
   stockham_pass<N, V, NS, INV, (NS == 1 ? NZJ : 16)>(v, lre, lim, t, tw);
   if constexpr (NS * R < N) fft_wg<N, V, INV, NZJ, NS * R>(v, lre, lim, t, tw);

@@ -57,8 +57,8 @@ __device__ __forceinline__ void wave_lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 }
 
-template <int R, bool INV>
-__device__ __forceinline__ void twiddle_mul(float2 (&a)[R], const float2* __restrict__ tw, int k, int step) {
+template <int R, bool INV, class TWT>
+__device__ __forceinline__ void twiddle_mul(float2 (&a)[R], TWT tw, int k, int step) {
   if constexpr (R > 1) {
     float2 w[R];
     twiddles<R, INV>(w, tw, k, step);
@@ -74,8 +74,8 @@ __device__ __forceinline__ void dft_any(float2 (&a)[R]) {
 
 // NT independent transforms at once (separate LDS regions re[n] / im[n]): twiddles are
 // generated once per stage for all of them and the workgroup barriers are shared.
-template <int R, bool INV, int NT>
-__device__ __forceinline__ void twiddle_mul_n(float2 (&a)[NT][R], const float2* __restrict__ tw, int k, int step) {
+template <int R, bool INV, int NT, class TWT>
+__device__ __forceinline__ void twiddle_mul_n(float2 (&a)[NT][R], TWT tw, int k, int step) {
   if constexpr (R > 1) {
     float2 w[R];
     twiddles<R, INV>(w, tw, k, step);
@@ -86,14 +86,13 @@ __device__ __forceinline__ void twiddle_mul_n(float2 (&a)[NT][R], const float2* 
   }
 }
 
-template <int N, bool INV, int NZJ, int NT>
+template <int N, bool INV, int NZJ, int NT, class TWT>
 __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
-                                              int t, const float2* __restrict__ tw) {
+                                              int t, TWT tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
-  const float2* twp = tw;
-  asm volatile("" : "+s"(twp));
+  const auto twp = opaque_offset(tw);
   // stage 1: radix-8 over i, twiddle W_N^{k1 t}
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft_sparse<8, NZJ, INV>(v[n], twp);
@@ -187,14 +186,13 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
 }
 
-template <int N, bool INV, int NT>
+template <int N, bool INV, int NT, class TWT>
 __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
-                                              int t, const float2* __restrict__ tw) {
+                                              int t, TWT tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
-  const float2* twp = tw;
-  asm volatile("" : "+s"(twp));
+  const auto twp = opaque_offset(tw);
   const int grp = l >> 3, rr = l & 7;
   // stage 4^T: radix-8 over s -> p
 #pragma unroll
@@ -288,18 +286,18 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
 }
 
 // single-transform forms
-template <int N, bool INV, int NZJ>
+template <int N, bool INV, int NZJ, class TWT>
 __device__ __forceinline__ void fs_nat2perm(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
-                                            const float2* __restrict__ tw) {
+                                            TWT tw) {
   float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
   float* const rr[1] = {re};
   float* const ii[1] = {im};
   fs_nat2perm_n<N, INV, NZJ, 1>(vv, rr, ii, t, tw);
 }
 
-template <int N, bool INV>
+template <int N, bool INV, class TWT>
 __device__ __forceinline__ void fs_perm2nat(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
-                                            const float2* __restrict__ tw) {
+                                            TWT tw) {
   float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
   float* const rr[1] = {re};
   float* const ii[1] = {im};

@@ -17,6 +17,7 @@
 // the row pass reads, whole 64-byte segments.
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -32,6 +33,7 @@ namespace tdtm {
 constexpr int V = 8;   // values per thread in every FFT
 constexpr int CW = 2;  // columns per column-pass block (tile width)
 constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
+constexpr int TDTM_LDS_BYTES = 160 * 1024;  // LDS one workgroup may allocate on gfx950
 
 thread_local std::string g_err;
 
@@ -347,33 +349,49 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
   }
 }
 
-// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
-// conjugate against the image spectrum column, inverse FFT along y; tiled store.
-// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
-// conjugate against the image spectrum column, inverse FFT along y; tiled store.
-// Sizes >= 512 use the four-step transform (the product is formed in its permuted
-// order, the image column is held in registers in that order); smaller sizes Stockham.
-// NT correlations of one column through the four-step transforms together (twiddles and
-// barriers shared): load the pruned samples, forward FFT, multiply by conj against the
-// image column (held in xi in the permuted order), inverse FFT, tiled store.
+// Inputs of NT correlations k, k+1, .. (clamped to klast, so a prefetch past the end of
+// the chunk re-reads its last correlation) for the four-step column transform: the
+// leading NL slots (rows t + i*T < hs) of column c; the remaining slots are zero.
 template <int HH, int NZJ, int NT>
-__device__ __forceinline__ void colpass_four(const float2* __restrict__ Rb, int hs, int G, int g, int c, int t,
-                                             int k, int ystore, bool dc, const float2 (&xi)[V],
-                                             float* const (&re)[NT], float* const (&im)[NT],
-                                             const float2* __restrict__ tw, float2* __restrict__ Gb) {
+__device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
+                                             int hs, int G, int g, int c, int t, int k, int klast) {
+  constexpr int T = HH / V;
+  constexpr int NL = NZJ < V ? NZJ : V;
+#pragma unroll
+  for (int n = 0; n < NT; ++n) {
+    const float2* rin = Rb + ((long)min(k + n, klast) * G + g) * hs * CW;
+#pragma unroll
+    for (int i = 0; i < NL; ++i) {
+      const int yy = t + i * T;
+      in[n][i] = yy < hs ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
+    }
+  }
+}
+
+// Column pass, four-step path: NT correlations k .. k+NT-1 of one column through the
+// transforms together (twiddles and barriers shared): pruned forward FFT of the inputs
+// in `in`, multiply by conj against the image column (held in xi in the permuted
+// order), inverse FFT, tiled store.  The inputs of the next iteration (NX correlations
+// from k + NT, clamped to klast) are loaded into `in` right after the forward
+// transforms, so their latency overlaps the inverse transforms and this iteration's
+// stores.
+template <int HH, int NZJ, int NT, int NX, bool FULL, class TWT>
+__device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
+                                             int hs, int G, int g, int c, int t, int k, int klast, int ystore, bool dc,
+                                             const float2 (&xi)[V], float* const (&re)[NT], float* const (&im)[NT],
+                                             TWT tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr int NL = NZJ < V ? NZJ : V;
   float2 v[NT][V];
 #pragma unroll
-  for (int n = 0; n < NT; ++n) {
-    const float2* rin = Rb + ((long)(k + n) * G + g) * hs * CW;
+  for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const int yy = t + i * T;
-      v[n][i] = (i < NL && yy < hs) ? rin[(long)yy * CW + c] : make_float2(0.f, 0.f);
-    }
-  }
+    for (int i = 0; i < V; ++i) v[n][i] = i < NL ? in[n][i] : make_float2(0.f, 0.f);
   fs_nat2perm_n<HH, false, NZJ, NT>(v, re, im, t, tw);
+  if constexpr (NX > 0) {
+    float2(&nx)[NX][NL] = *reinterpret_cast<float2(*)[NX][NL]>(&in);
+    colpass_load<HH, NZJ, NX>(nx, Rb, hs, G, g, c, t, k + NT, klast);
+  }
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
 #pragma unroll
@@ -381,14 +399,14 @@ __device__ __forceinline__ void colpass_four(const float2* __restrict__ Rb, int 
     if (dc) v[n][0] = make_float2(0.f, 0.f);
   }
   fs_perm2nat_n<HH, true, NT>(v, re, im, t, tw);
+  // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR
+  constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float2* gout = Gb + (long)(k + n) * HH * G * CW;
+    float2* gout = Gb + (long)(k + n) * HH * G * CW + ((long)(t / TR) * G + g) * TR * CW + (t % TR) * CW + c;
 #pragma unroll
-    for (int i = 0; i < V; ++i) {
-      const int yy = t + i * T;
-      if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[n][i];
-    }
+    for (int i = 0; i < V; ++i)
+      if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * G] = v[n][i];
   }
 }
 
@@ -398,17 +416,40 @@ __device__ __forceinline__ void colpass_four(const float2* __restrict__ Rb, int 
 // order, the image column is held in registers in that order) with NT correlations per
 // iteration; smaller sizes use the Stockham passes.  CWB = columns per block (1 or the
 // CW columns of a tile).
-template <int HH, int NZJ, int CWB, int NT>
+// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
+// conjugate against the image spectrum column, inverse FFT along y; tiled store.
+// Sizes >= 512 use the four-step transform with NT correlations per iteration; smaller
+// sizes use the Stockham passes.  CWB = columns per block (1 or the CW columns of a
+// tile).  TWM = 1 reads the twiddles from an LDS copy of the table entries the
+// four-step transforms of size HH use; FULL = every row is stored (ystore == HH).
+template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int K, int kchunk, int ystore,
-                                                         const float2* __restrict__ tw, float2* __restrict__ Gb) {
+                                                         const float2* __restrict__ tw_global, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr bool FOUR = UseFour<HH>::value;
   constexpr int HF = FOUR ? HH : 512;
   constexpr int LF = FOUR ? FS<HF>::floats : LdsSize<HH>::floats;
+  constexpr int NL = NZJ < V ? NZJ : V;
   static_assert(CWB == CW || (CWB == 1 && FOUR), "one column per block needs the four-step path");
   static_assert(NT == 1 || FOUR, "paired correlations need the four-step path");
-  __shared__ float lds[CWB * NT * 2 * LF];
+  static_assert(TWM == 0 || FOUR, "the LDS twiddle table serves the four-step path");
+  // LDS copy: entries j * (TW_N / HH) for j < HH / 2 -- the four-step transforms of size
+  // HH read only multiples of TW_N / HH below TW_N / 2
+  __shared__ float lds[CWB * NT * 2 * LF + (TWM ? HH : 0)];
+  float2* ltw = reinterpret_cast<float2*>(lds + CWB * NT * 2 * LF);
+  constexpr int SH = tw_shift(HH);
+  static_assert((HH << SH) == TW_N, "column length must divide the twiddle table");
+  if constexpr (TWM == 1) {
+    for (int j = threadIdx.x; j < HH / 2; j += CWB * T) ltw[j] = tw_global[j << SH];
+    __syncthreads();
+  }
+  const auto tw = [&]() {
+    if constexpr (TWM == 1)
+      return TwLds<SH>{ltw};
+    else
+      return tw_global;
+  }();
   // four-step: the waves of one column are contiguous; Stockham: columns interleaved
   const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
   const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
@@ -428,13 +469,39 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   if constexpr (FOUR) {
     float* const re1[1] = {base};
     float* const im1[1] = {base + LF};
+    float2 in[NT][NL];
+    float2(&in1)[1][NL] = *reinterpret_cast<float2(*)[1][NL]>(&in);
     int k = k0;
     if constexpr (NT > 1) {
       float* const re2[2] = {base, base + 2 * LF};
       float* const im2[2] = {base + LF, base + 3 * LF};
-      for (; k + 1 < k1; k += 2) colpass_four<HF, NZJ, 2>(Rb, hs, G, g, c, t, k, ystore, dc, xi, re2, im2, tw, Gb);
+      const int klast = k1 - 1;
+      // the first iteration is peeled so that the loop is always entered with the
+      // prefetched inputs followed by one iteration's stores in the memory queue (the
+      // wait for the inputs then leaves the stores in flight)
+      if (k + 1 < k1) {
+        colpass_load<HF, NZJ, 2>(in, Rb, hs, G, g, c, t, k, klast);
+        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+        k += 2;
+      }
+      for (; k + 1 < k1; k += 2)
+        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+      // a single last correlation: its input was prefetched into in[0], unless the chunk
+      // holds only this one
+      if (k < k1) {
+        if (k == k0) colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
+        colpass_four<HF, NZJ, 1, 0, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+      }
+    } else {
+      const int klast = k1 - 1;
+      if (k < k1) {  // peeled first iteration (see the paired loop)
+        colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
+        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        ++k;
+      }
+      for (; k < k1; ++k)
+        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
     }
-    for (; k < k1; ++k) colpass_four<HF, NZJ, 1>(Rb, hs, G, g, c, t, k, ystore, dc, xi, re1, im1, tw, Gb);
   } else {
     float* lre = base;
     float* lim = base + LF;
@@ -455,7 +522,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
 #pragma unroll
       for (int i = 0; i < V; ++i) {
         const int yy = t + i * T;
-        if (yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+        if (FULL || yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
       }
     }
   }
@@ -536,6 +603,37 @@ __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, 
   }
 }
 
+// Synchronous tile staging with every load of the thread issued before the first LDS
+// store (one memory latency per tile instead of one per round).
+template <int WS>
+__device__ __forceinline__ void tile_block_sync_batched(const float2* __restrict__ src, float* dst) {
+  using C = RowCfg<WS>;
+  constexpr int NF4 = C::TILE / 2;
+  constexpr int BLOCK = TR * C::T;
+  constexpr int ROUNDS = (NF4 + BLOCK - 1) / BLOCK;
+  const float4* s4 = reinterpret_cast<const float4*>(src);
+  float4 x[ROUNDS];
+#pragma unroll
+  for (int j = 0; j < ROUNDS; ++j) {
+    const int f = threadIdx.x + j * BLOCK;
+    if (j + 1 < ROUNDS || f < NF4) x[j] = s4[f];
+  }
+#pragma unroll
+  for (int j = 0; j < ROUNDS; ++j) {
+    const int f = threadIdx.x + j * BLOCK;
+    if (j + 1 < ROUNDS || f < NF4) {
+      const int g = f / TR, rr = f % TR;
+      const int a = stg_pad(2 * g);
+      float* re = dst + rr * C::RSTR;
+      float* im = re + C::LF;
+      re[a] = x[j].x;
+      re[a + 1] = x[j].z;
+      im[a] = x[j].y;
+      im[a + 1] = x[j].w;
+    }
+  }
+}
+
 // Reformat of a raw staged tile block (HBM layout) into the de-interleaved per-row
 // re / im regions: consecutive 16-byte reads, paired 8-byte writes.
 template <int WS>
@@ -568,18 +666,40 @@ __device__ __forceinline__ void tile_block_reformat(const float* raw, float* dst
 // de-interleaved regions.
 template <int WS, int MODE>
 struct RowLds {
-  static constexpr int floats = MODE == 0 ? RowCfg<WS>::BUF
-                              : MODE == 1 ? 2 * RowCfg<WS>::BUF
-                                          : 2 * RowCfg<WS>::TILE + RowCfg<WS>::REGION;
+  static constexpr int floats = (MODE == 0 || MODE == 3) ? RowCfg<WS>::BUF
+                              : MODE == 1                ? 2 * RowCfg<WS>::BUF
+                                                         : 2 * RowCfg<WS>::TILE + RowCfg<WS>::REGION;
 };
 
-template <int WS, int MODE, class Epi>
+// log2(TW_N / WS): a W-point row pass reads only table entries that are multiples of
+// TW_N / W, so its LDS twiddle copy holds W entries.
+template <int WS>
+struct RowTw {
+  static constexpr int SH = tw_shift(WS);
+  static_assert((WS << SH) == TW_N, "row length must divide the twiddle table");
+  static constexpr int floats = 2 * WS;
+};
+
+template <int WS, int MODE, int TWM, class Epi>
 __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p,
-                                             const float2* __restrict__ tw, float* lds, Epi&& epi) {
+                                             const float2* __restrict__ tw_global, float* lds, Epi&& epi) {
   using C = RowCfg<WS>;
   constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
   constexpr bool ASYNC = MODE == 1;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
+  // twiddle source: the global table, or (TWM == 1) a copy of its every (TW_N / WS)-th
+  // entry placed after the staging / work area
+  float2* ltw = reinterpret_cast<float2*>(lds + RowLds<WS, MODE>::floats);
+  if constexpr (TWM == 1) {
+    for (int j = threadIdx.x; j < WS; j += TR * T) ltw[j] = tw_global[j << RowTw<WS>::SH];
+    __syncthreads();
+  }
+  const auto tw = [&]() {
+    if constexpr (TWM == 1)
+      return TwLds<RowTw<WS>::SH>{ltw};
+    else
+      return tw_global;
+  }();
   const long kstride = (long)(HH / TR) * C::TILE;
   const float2* gsrc = Gb + (long)p * C::TILE;
   float* raw2 = lds;                  // MODE 2: raw tile
@@ -599,6 +719,9 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       tile_block_reformat<WS>(raw2, work2);
       __syncthreads();                // raw consumed, work2 staged
       if (k + 1 < K) tile_block_async<WS>(gsrc + (k + 1) * kstride, raw2);
+    } else if (MODE == 3) {
+      tile_block_sync_batched<WS>(gsrc + k * kstride, cur);
+      __syncthreads();
     } else {
       tile_block_sync<WS>(gsrc + k * kstride, cur);
       __syncthreads();
@@ -607,8 +730,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
     const float* sre = cur + r * C::RSTR;
     const float* sim = sre + LF;
     float2 v[V];
-    const float2* twp = tw;
-    asm volatile("" : "+s"(twp));  // reload the pre-twiddles per correlation (no hoisting)
+    const auto twp = opaque_offset(tw);  // reload the pre-twiddles per correlation (no hoisting)
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       int mm = t + i * T;
@@ -646,7 +768,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
 
 // Row pass of the search: fused MIP / argmax / sum / sum-of-squares over the K
 // correlations of the step, then one merge into the running maps.
-template <int WS, int MODE>
+template <int WS, int MODE, int TWM>
 __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
                                                             int nb, int nd, int O, int o0,
                                                             const float2* __restrict__ tw, float* __restrict__ mip,
@@ -655,7 +777,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
                                                             float* __restrict__ qlo) {
   using C = RowCfg<WS>;
   constexpr int T = C::T;
-  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE>::floats];
+  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE>::floats + (TWM ? RowTw<WS>::floats : 0)];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int p = blockIdx.x;  // row-tile index
   const int y = p * TR + r;
@@ -672,7 +794,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
-  rowpass_loop<WS, MODE>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, MODE, TWM>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -827,7 +949,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
                                                                    RefineBest* __restrict__ part) {
   using C = RowCfg<WS>;
   constexpr int T = C::T;
-  __shared__ __attribute__((aligned(16))) float lds[C::BUF];
+  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, 3>::floats + RowTw<WS>::floats];
   __shared__ RefineBest red[TR * T];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int p = blockIdx.x;
@@ -840,7 +962,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  rowpass_loop<WS, 0>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, 3, 1>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -1001,9 +1123,16 @@ struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
-  int opt_colpass_pair = 1;     // two correlations per column-pass iteration
+  int opt_colpass_pair = 0;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
-  int opt_rowpass_staging = 0;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat
+  int opt_colpass_twiddles = 1; // four-step column transforms: 0 global table, 1 LDS copy
+  int opt_rowpass_staging = 3;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat,
+                                // 3: synchronous de-interleaved with batched loads
+  int opt_rowpass_twiddles = 1; // 0: twiddles from the global table, 1: LDS copy (staging 0 / 3 only)
+  int opt_phase_timing = 0;     // record HIP events between the phases of every step
+  // per-step events at the phase boundaries (project | rowfwd | colpass | rowpass), read and
+  // released by tdtm_match_plan_phase_times
+  std::vector<std::array<hipEvent_t, TDTM_NUM_PHASES + 1>> phase_events;
   float2* tw = nullptr;
   float2* Xt = nullptr;
   float* filt = nullptr;
@@ -1031,7 +1160,14 @@ static int dalloc(T** p, size_t count, int64_t& acc) {
   return 0;
 }
 
+static void release_phase_events(tdtm_match_plan* p) {
+  for (auto& ev : p->phase_events)
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  p->phase_events.clear();
+}
+
 static void plan_free(tdtm_match_plan* p) {
+  release_phase_events(p);
   void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb, p->ctfbuf, p->rpart,
                   p->rslots};
   for (void* b : bufs)
@@ -1092,8 +1228,24 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
   nchunk = (K + kchunk - 1) / kchunk;
   dim3 grid(nblk_x, nchunk);
-#define TDTM_COLPASS(CWB_, NT_) \
-  k_colpass<HH, NZJ, CWB_, NT_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, ystore, p->tw, p->Gb)
+  // the four-step LDS twiddle copy (HH / 2 entries) when it fits next to the transforms
+  constexpr int LFc = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
+  const bool full = ystore == HH;
+  const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
+#define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                                      \
+  k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, \
+                                                                               ystore, p->tw, p->Gb)
+#define TDTM_COLPASS(CWB_, NT_)                                                           \
+  do {                                                                                    \
+    constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + HH) * 4 <= TDTM_LDS_BYTES;      \
+    if (ltw && fits_) {                                                                   \
+      if (full) TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), true);                         \
+      else TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), false);                             \
+    } else {                                                                              \
+      if (full) TDTM_COLPASS_T(CWB_, NT_, 0, true);                                       \
+      else TDTM_COLPASS_T(CWB_, NT_, 0, false);                                           \
+    }                                                                                     \
+  } while (0)
   if constexpr (FOUR) {
     if (cols == 1 && nt == 2) TDTM_COLPASS(1, 2);
     else if (cols == 1) TDTM_COLPASS(1, 1);
@@ -1103,6 +1255,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
     TDTM_COLPASS(CW, 1);
   }
 #undef TDTM_COLPASS
+#undef TDTM_COLPASS_T
 }
 
 template <int HH>
@@ -1114,20 +1267,28 @@ static void launch_colpass(tdtm_match_plan* p, int K, int ystore, hipStream_t st
   else launch_colpass_n<HH, 8>(p, K, ystore, st);
 }
 
+template <int WS, int MODE, int TWM>
+static void launch_rowpass_mt(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
+                              float* slo, float* qhi, float* qlo, hipStream_t st) {
+  k_rowpass<WS, MODE, TWM><<<dim3(p->H / TR), TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0,
+                                                                         p->tw, mip, idx, shi, slo, qhi, qlo);
+}
+
 template <int WS>
 static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                            float* slo, float* qhi, float* qlo, hipStream_t st) {
-  dim3 grid(p->H / TR);
-  const int threads = TR * (WS / 2) / V;
-  if (p->opt_rowpass_staging == 1)
-    k_rowpass<WS, 1><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
-                                               qlo);
-  else if (p->opt_rowpass_staging == 2)
-    k_rowpass<WS, 2><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
-                                               qlo);
-  else
-    k_rowpass<WS, 0><<<grid, threads, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi,
-                                               qlo);
+  const bool ltw = p->opt_rowpass_twiddles == 1;
+  switch (p->opt_rowpass_staging) {
+    case 1: launch_rowpass_mt<WS, 1, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st); break;
+    case 2: launch_rowpass_mt<WS, 2, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st); break;
+    case 3:
+      if (ltw) launch_rowpass_mt<WS, 3, 1>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+      else launch_rowpass_mt<WS, 3, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+      break;
+    default:
+      if (ltw) launch_rowpass_mt<WS, 0, 1>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+      else launch_rowpass_mt<WS, 0, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+  }
 }
 
 template <int WS>
@@ -1195,10 +1356,27 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
     delete p;
     return 1;
   }
+  // tw[m] = e^{-2 pi i m / TW_N}: evaluated on the first octant and mapped by symmetry,
+  // so quarter and eighth turns are exact and the table has the circle's symmetries
   std::vector<float2> tw(TW_N);
+  constexpr int Q = TW_N / 4;
   for (int m = 0; m < TW_N; ++m) {
-    const double a = -2.0 * M_PI * (double)m / (double)TW_N;
-    tw[m] = make_float2((float)std::cos(a), (float)std::sin(a));
+    const int q = m / Q, r = m % Q;
+    double c, sn;  // cos, sin of 2 pi r / TW_N, r in [0, Q)
+    if (2 * r <= Q) {
+      const double a = 2.0 * M_PI * (double)r / (double)TW_N;
+      c = std::cos(a);
+      sn = std::sin(a);
+    } else {
+      const double a = 2.0 * M_PI * (double)(Q - r) / (double)TW_N;
+      c = std::sin(a);
+      sn = std::cos(a);
+    }
+    double cq = c, sq = sn;  // rotate by q quarter turns
+    if (q == 1) cq = -sn, sq = c;
+    if (q == 2) cq = -c, sq = -sn;
+    if (q == 3) cq = sn, sq = -c;
+    tw[m] = make_float2((float)cq, (float)-sq);
   }
   hipError_t e = hipMemcpy(p->tw, tw.data(), TW_N * sizeof(float2), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -1227,12 +1405,25 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     p->opt_colpass_columns = value;
   } else if (k == "colpass_pair") {
     p->opt_colpass_pair = value != 0;
+  } else if (k == "colpass_twiddles") {
+    if (value < 0 || value > 1) return fail("colpass_twiddles must be 0 or 1");
+    p->opt_colpass_twiddles = value;
   } else if (k == "colpass_chunks") {
     if (value < 0) return fail("colpass_chunks must be >= 0");
     p->opt_colpass_chunks = value;
   } else if (k == "rowpass_staging") {
-    if (value < 0 || value > 2) return fail("rowpass_staging must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail("rowpass_staging must be 0, 1, 2 or 3");
+    if ((value == 1 || value == 2) && p->opt_rowpass_twiddles)
+      return fail("rowpass_staging 1 / 2 leave no LDS for rowpass_twiddles = 1");
     p->opt_rowpass_staging = value;
+  } else if (k == "rowpass_twiddles") {
+    if (value < 0 || value > 1) return fail("rowpass_twiddles must be 0 or 1");
+    if (value && (p->opt_rowpass_staging == 1 || p->opt_rowpass_staging == 2))
+      return fail("rowpass_twiddles = 1 needs rowpass_staging 0 or 3");
+    p->opt_rowpass_twiddles = value;
+  } else if (k == "phase_timing") {
+    p->opt_phase_timing = value != 0;
+    if (!p->opt_phase_timing) release_phase_events(p);
   } else {
     return fail("unknown option '" + k + "'");
   }
@@ -1290,6 +1481,14 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   hipStream_t st = (hipStream_t)stream;
   const int K = count * p->D * p->C;
   if (K >= 65535) return fail("orientations x defocus x pixel sizes per step must be < 65535");
+  hipEvent_t* ev = nullptr;
+  if (p->opt_phase_timing) {
+    std::array<hipEvent_t, TDTM_NUM_PHASES + 1> e{};
+    for (auto& x : e) TDTM_HIP(hipEventCreate(&x));
+    p->phase_events.push_back(e);
+    ev = p->phase_events.back().data();
+    TDTM_HIP(hipEventRecord(ev[0], st));
+  }
   const long ns = (long)count * p->h * p->wr;
   k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S);
   TDTM_LAUNCHED();
@@ -1302,13 +1501,43 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   TDTM_LAUNCHED();
   k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->H, p->W, p->ae);
   TDTM_LAUNCHED();
+  if (ev) TDTM_HIP(hipEventRecord(ev[1], st));
   TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
   TDTM_LAUNCHED();
+  if (ev) TDTM_HIP(hipEventRecord(ev[2], st));
   TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, p->H, st);
   TDTM_LAUNCHED();
+  if (ev) TDTM_HIP(hipEventRecord(ev[3], st));
   TDTM_SWITCH_POW2(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
   TDTM_LAUNCHED();
+  if (ev) TDTM_HIP(hipEventRecord(ev[4], st));
   return 0;
+}
+
+int tdtm_match_plan_phase_times(tdtm_match_plan_t p, double* out_ms, int* out_steps) {
+  if (!p || !out_ms || !out_steps) return fail("null argument");
+  for (int i = 0; i < TDTM_NUM_PHASES; ++i) out_ms[i] = 0.0;
+  *out_steps = (int)p->phase_events.size();
+  int rc = 0;
+  for (auto& e : p->phase_events) {
+    if (rc) break;
+    hipError_t err = hipEventSynchronize(e[TDTM_NUM_PHASES]);
+    if (err != hipSuccess) {
+      rc = fail(std::string("hipEventSynchronize: ") + hipGetErrorString(err));
+      break;
+    }
+    for (int i = 0; i < TDTM_NUM_PHASES; ++i) {
+      float ms = 0.f;
+      err = hipEventElapsedTime(&ms, e[i], e[i + 1]);
+      if (err != hipSuccess) {
+        rc = fail(std::string("hipEventElapsedTime: ") + hipGetErrorString(err));
+        break;
+      }
+      out_ms[i] += ms;
+    }
+  }
+  release_phase_events(p);
+  return rc;
 }
 
 int tdtm_match_finalize(int64_t n, int64_t total, const float* mip, const int32_t* idx, const float* shi,

@@ -26,6 +26,7 @@ SIGNATURES = {
     "tdtm_match_plan_destroy": (c_int, [c_void_p]),
     "tdtm_match_plan_bytes": (c_int64, [c_void_p]),
     "tdtm_match_plan_set_option": (c_int, [c_void_p, c_char_p, c_int]),
+    "tdtm_match_plan_phase_times": (c_int, [c_void_p, c_void_p, c_void_p]),
     "tdtm_match_set_image": (c_int, [c_void_p, c_void_p, c_void_p]),
     "tdtm_match_set_filters": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tdtm_stats_init": (c_int, [c_int64] + [c_void_p] * 6 + [c_void_p]),

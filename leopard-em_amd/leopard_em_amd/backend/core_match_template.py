@@ -15,6 +15,8 @@ output dictionary.  Differences, all deliberate:
   num_defocus * num_pixel > 1).
 * ``num_cuda_streams`` and ``backend`` are accepted for signature compatibility; all
   work of a step is one fused launch sequence on the current stream.
+* with the environment variable ``TDTM_AUTOTUNE=1`` each device's plan picks its
+  launch shapes with ``backend.autotune`` (cached per shape and device).
 """
 
 from __future__ import annotations
@@ -70,6 +72,16 @@ class MatchPlan:
     def set_option(self, key: str, value: int) -> None:
         """Launch-shape option (see tdtm_match_plan_set_option); results are unaffected."""
         _lib.check(self.lib.tdtm_match_plan_set_option(self.handle, key.encode(), int(value)))
+
+    PHASES = ("project", "rowfwd", "colpass", "rowpass")
+
+    def phase_times(self) -> tuple[dict[str, float], int]:
+        """({phase: device ms summed over steps}, steps) for the steps recorded since the
+        last call; needs ``set_option("phase_timing", 1)``."""
+        out = (_lib.ctypes.c_double * len(self.PHASES))()
+        n = _lib.c_int()
+        _lib.check(self.lib.tdtm_match_plan_phase_times(self.handle, out, _lib.ctypes.byref(n)))
+        return dict(zip(self.PHASES, list(out))), int(n.value)
 
     def _stream(self):
         return _lib.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -218,6 +230,10 @@ def core_match_template(
             tdft = template_dft.to(dev, torch.complex64).contiguous()
             eul = euler_angles.to(dev, torch.float32).contiguous()
             plan.reset_stats()
+            if os.environ.get("TDTM_AUTOTUNE", "0") == "1" and hi > lo:
+                from .autotune import autotune
+
+                autotune(plan, tdft, eul[lo:hi].contiguous())
             if hi > lo:
                 plan.run(tdft, eul, lo, hi)
             plan._tdft, plan._eul = tdft, eul

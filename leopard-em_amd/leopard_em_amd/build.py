@@ -22,6 +22,9 @@ HIPCC_FLAGS = [
     "-shared",
     # no packed FP32 (v_pk_*_f32) code generation
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # fused multiply-adds only where the source writes fmaf: rounding is the same in
+    # every code shape, so launch options cannot change results
+    "-ffp-contract=off",
 ]
 
 

@@ -151,8 +151,11 @@ def test_match_peak_list_identical(cuda_device):
 
 
 @pytest.mark.parametrize("opts", [
-    {"colpass_columns": 1}, {"colpass_pair": 0}, {"rowpass_staging": 1}, {"rowpass_staging": 2}, {"colpass_chunks": 3},
-    {"colpass_columns": 1, "colpass_pair": 0},
+    {"colpass_columns": 1}, {"colpass_pair": 1}, {"rowpass_staging": 0}, {"rowpass_twiddles": 0},
+    {"rowpass_twiddles": 0, "rowpass_staging": 0}, {"rowpass_twiddles": 0, "rowpass_staging": 1},
+    {"rowpass_twiddles": 0, "rowpass_staging": 2}, {"colpass_chunks": 3}, {"colpass_columns": 1, "colpass_pair": 0},
+    {"colpass_twiddles": 0}, {"colpass_twiddles": 0, "colpass_pair": 1}, {"colpass_columns": 1, "colpass_chunks": 5},
+    {"phase_timing": 1},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
     from leopard_em_amd.backend import MatchPlan, raw_statistics
@@ -182,3 +185,29 @@ def test_unknown_launch_option_raises(cuda_device):
         plan.set_option("no_such_option", 1)
     with pytest.raises(_lib.TdtmError):
         plan.set_option("colpass_columns", 3)
+    plan.set_option("rowpass_staging", 0)
+    plan.set_option("rowpass_twiddles", 1)
+    with pytest.raises(_lib.TdtmError, match="LDS"):
+        plan.set_option("rowpass_staging", 1)
+    plan.set_option("rowpass_twiddles", 0)
+    plan.set_option("rowpass_staging", 2)
+    with pytest.raises(_lib.TdtmError, match="rowpass_staging"):
+        plan.set_option("rowpass_twiddles", 1)
+
+
+def test_phase_times(cuda_device):
+    from leopard_em_amd.backend import MatchPlan
+
+    args = _inputs(512, 512, 64, 2, 1, 8, seed=5)
+    img_dft, tdft, ctf, wf, dfs, pix, eul = args
+    plan = MatchPlan(512, 512, 64, 4, 2, 1, cuda_device)
+    plan.set_inputs(img_dft, ctf, wf)
+    plan.reset_stats()
+    plan.set_option("phase_timing", 1)
+    plan.run(tdft.to(cuda_device), eul.to(cuda_device), 0, 8)
+    times, n = plan.phase_times()
+    assert n == 2
+    assert set(times) == {"project", "rowfwd", "colpass", "rowpass"}
+    assert all(v > 0 for v in times.values())
+    times, n = plan.phase_times()  # events were released
+    assert n == 0 and all(v == 0 for v in times.values())
