@@ -60,8 +60,9 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
  * for the four-step column transforms where it fits, default), "rowpass_staging" (0: synchronous de-interleaved,
  * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat,
  * 3: synchronous de-interleaved with batched loads; default 3), "rowpass_twiddles" (0: global
- * table, 1: LDS copy, default; needs staging 0 or 3), "phase_timing" (0/1: record HIP events
- * between the phases of every step).
+ * table, 1: LDS copy, default; needs staging 0 or 3), "rowpass_rows" (rows of a 4-row tile
+ * row per row-pass block: 1 (default), 2 or 4; below 4 needs staging 0 or 3), "phase_timing"
+ * (0/1: record HIP events between the phases of every step).
  * Results do not depend on the options.
  */
 int tdtm_match_plan_set_option(tdtm_match_plan_t plan, const char* key, int value);

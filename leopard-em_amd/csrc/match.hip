@@ -437,6 +437,20 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   // LDS copy: entries j * (TW_N / HH) for j < HH / 2 -- the four-step transforms of size
   // HH read only multiples of TW_N / HH below TW_N / 2
   __shared__ float lds[CWB * NT * 2 * LF + (TWM ? HH : 0)];
+  // block -> (tile column g, column c): with one column per block, the two columns of a
+  // tile column get block ids 8 apart, so they run on the same XCD (workgroups go to the
+  // 8 XCDs round robin) and their halves of every 64-byte tile meet in its L2; the grid
+  // is padded to whole groups of 16 and the blocks past the last tile column exit
+  int g, c;
+  if constexpr (CWB == CW) {
+    g = blockIdx.x;
+    c = 0;
+  } else {
+    const int b = blockIdx.x, rest = b >> 3;
+    c = rest & 1;
+    g = (rest >> 1) * 8 + (b & 7);
+    if (g >= G) return;
+  }
   float2* ltw = reinterpret_cast<float2*>(lds + CWB * NT * 2 * LF);
   constexpr int SH = tw_shift(HH);
   static_assert((HH << SH) == TW_N, "column length must divide the twiddle table");
@@ -453,8 +467,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   // four-step: the waves of one column are contiguous; Stockham: columns interleaved
   const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
   const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
-  const int g = CWB == CW ? (int)blockIdx.x : (int)blockIdx.x >> 1;
-  const int c = CWB == CW ? cl : (int)blockIdx.x & 1;
+  if constexpr (CWB == CW) c = cl;
   const int k0 = blockIdx.y * kchunk;
   const int k1 = min(K, k0 + kchunk);
   float2 xi[V];
@@ -536,18 +549,22 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
   lo += err;
 }
 
-// Row-pass geometry for a W-point row (M = W/2 complex points).
-template <int WS>
+// Row-pass geometry for a W-point row (M = W/2 complex points) with RB of the TR rows of
+// a tile row per block.
+template <int WS, int RB = TR>
 struct RowCfg {
+  static_assert(RB == 1 || RB == 2 || RB == 4, "rows per block");
   static constexpr int M = WS / 2;
   static constexpr int T = M / V;
+  static constexpr int BLOCK = RB * T;                                     // threads per block
   static constexpr bool FOUR = UseFour<M>::value;
   static constexpr int MF = FOUR ? M : 512;
   static constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;  // per component per row
   static constexpr int G = (M + 1 + CW - 1) / CW;                          // tile columns
-  static constexpr int TILE = G * TR * CW;                                 // complex per row-tile block
+  static constexpr int TILEF = G * TR * CW;                                // complex per tile row in HBM
+  static constexpr int TILE = G * RB * CW;                                 // complex staged per block
   static constexpr int RSTR = 2 * LF + 16;                                 // floats per row region
-  static constexpr int REGION = TR * RSTR;                                 // floats, FFT work area
+  static constexpr int REGION = RB * RSTR;                                 // floats, FFT work area
   static constexpr int BUF = REGION > 2 * TILE ? REGION : 2 * TILE;        // floats per LDS buffer
 };
 
@@ -584,15 +601,17 @@ __device__ __forceinline__ void tile_block_async(const float2* __restrict__ src,
 // thread (row r, t) holds v[i] = (x[2n], x[2n+1]) for n = t + i*T of row y = p*TR + r.
 // Synchronous tile staging: 16-byte loads (two columns of one row), stored de-interleaved
 // into the per-row split re / im regions at stg_pad positions.
-template <int WS>
+// `src` points at the block's first row inside the tile row; a 16-byte chunk is one row's
+// two columns of one tile, chunks of consecutive tiles are TR chunks apart.
+template <int WS, int RB>
 __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, float* dst) {
-  using C = RowCfg<WS>;
+  using C = RowCfg<WS, RB>;
   constexpr int NF4 = C::TILE / 2;
-  constexpr int BLOCK = TR * C::T;
+  constexpr int BLOCK = C::BLOCK;
   const float4* s4 = reinterpret_cast<const float4*>(src);
   for (int f = threadIdx.x; f < NF4; f += BLOCK) {
-    const float4 x = s4[f];
-    const int g = f / TR, rr = f % TR;
+    const int g = f / RB, rr = f % RB;
+    const float4 x = s4[g * TR + rr];
     const int a = stg_pad(2 * g);
     float* re = dst + rr * C::RSTR;
     float* im = re + C::LF;
@@ -605,24 +624,25 @@ __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, 
 
 // Synchronous tile staging with every load of the thread issued before the first LDS
 // store (one memory latency per tile instead of one per round).
-template <int WS>
+template <int WS, int RB>
 __device__ __forceinline__ void tile_block_sync_batched(const float2* __restrict__ src, float* dst) {
-  using C = RowCfg<WS>;
+  using C = RowCfg<WS, RB>;
   constexpr int NF4 = C::TILE / 2;
-  constexpr int BLOCK = TR * C::T;
+  constexpr int BLOCK = C::BLOCK;
   constexpr int ROUNDS = (NF4 + BLOCK - 1) / BLOCK;
+  static_assert((ROUNDS - 1) * BLOCK < NF4 && ROUNDS * BLOCK >= NF4, "only the last round is partial");
   const float4* s4 = reinterpret_cast<const float4*>(src);
   float4 x[ROUNDS];
 #pragma unroll
   for (int j = 0; j < ROUNDS; ++j) {
     const int f = threadIdx.x + j * BLOCK;
-    if (j + 1 < ROUNDS || f < NF4) x[j] = s4[f];
+    if (j + 1 < ROUNDS || f < NF4) x[j] = s4[(f / RB) * TR + f % RB];
   }
 #pragma unroll
   for (int j = 0; j < ROUNDS; ++j) {
     const int f = threadIdx.x + j * BLOCK;
     if (j + 1 < ROUNDS || f < NF4) {
-      const int g = f / TR, rr = f % TR;
+      const int g = f / RB, rr = f % RB;
       const int a = stg_pad(2 * g);
       float* re = dst + rr * C::RSTR;
       float* im = re + C::LF;
@@ -664,34 +684,37 @@ __device__ __forceinline__ void tile_block_reformat(const float* raw, float* dst
 // per-row regions; 1 = asynchronous raw copy (double-buffered) read in place; 2 =
 // asynchronous raw copy of the next tile while the current one is reformatted into the
 // de-interleaved regions.
-template <int WS, int MODE>
+template <int WS, int MODE, int RB = TR>
 struct RowLds {
-  static constexpr int floats = (MODE == 0 || MODE == 3) ? RowCfg<WS>::BUF
+  static_assert(RB == TR || MODE == 0 || MODE == 3, "asynchronous staging copies whole tile rows");
+  static constexpr int floats = (MODE == 0 || MODE == 3) ? RowCfg<WS, RB>::BUF
                               : MODE == 1                ? 2 * RowCfg<WS>::BUF
                                                          : 2 * RowCfg<WS>::TILE + RowCfg<WS>::REGION;
 };
 
 // log2(TW_N / WS): a W-point row pass reads only table entries that are multiples of
-// TW_N / W, so its LDS twiddle copy holds W entries.
+// TW_N / W and below TW_N / 2 (pre-twiddles m < W/2, transform twiddles of the W/2-point
+// four-step), so its LDS twiddle copy holds W / 2 entries.
 template <int WS>
 struct RowTw {
   static constexpr int SH = tw_shift(WS);
   static_assert((WS << SH) == TW_N, "row length must divide the twiddle table");
-  static constexpr int floats = 2 * WS;
+  static constexpr int entries = WS / 2;
+  static constexpr int floats = 2 * entries;
 };
 
-template <int WS, int MODE, int TWM, class Epi>
-__device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p,
+template <int WS, int MODE, int TWM, int RB, class Epi>
+__device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p, int r0,
                                              const float2* __restrict__ tw_global, float* lds, Epi&& epi) {
-  using C = RowCfg<WS>;
+  using C = RowCfg<WS, RB>;
   constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
   constexpr bool ASYNC = MODE == 1;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   // twiddle source: the global table, or (TWM == 1) a copy of its every (TW_N / WS)-th
   // entry placed after the staging / work area
-  float2* ltw = reinterpret_cast<float2*>(lds + RowLds<WS, MODE>::floats);
+  float2* ltw = reinterpret_cast<float2*>(lds + RowLds<WS, MODE, RB>::floats);
   if constexpr (TWM == 1) {
-    for (int j = threadIdx.x; j < WS; j += TR * T) ltw[j] = tw_global[j << RowTw<WS>::SH];
+    for (int j = threadIdx.x; j < RowTw<WS>::entries; j += C::BLOCK) ltw[j] = tw_global[j << RowTw<WS>::SH];
     __syncthreads();
   }
   const auto tw = [&]() {
@@ -700,8 +723,8 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
     else
       return tw_global;
   }();
-  const long kstride = (long)(HH / TR) * C::TILE;
-  const float2* gsrc = Gb + (long)p * C::TILE;
+  const long kstride = (long)(HH / TR) * C::TILEF;
+  const float2* gsrc = Gb + (long)p * C::TILEF + r0 * CW;
   float* raw2 = lds;                  // MODE 2: raw tile
   float* work2 = lds + 2 * C::TILE;   // MODE 2: de-interleaved regions / transform work area
   if (MODE == 1 && K > 0) tile_block_async<WS>(gsrc, lds);
@@ -720,10 +743,10 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       __syncthreads();                // raw consumed, work2 staged
       if (k + 1 < K) tile_block_async<WS>(gsrc + (k + 1) * kstride, raw2);
     } else if (MODE == 3) {
-      tile_block_sync_batched<WS>(gsrc + k * kstride, cur);
+      tile_block_sync_batched<WS, RB>(gsrc + k * kstride, cur);
       __syncthreads();
     } else {
-      tile_block_sync<WS>(gsrc + k * kstride, cur);
+      tile_block_sync<WS, RB>(gsrc + k * kstride, cur);
       __syncthreads();
     }
     const float2* stg = reinterpret_cast<const float2*>(cur);
@@ -768,8 +791,27 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
 
 // Row pass of the search: fused MIP / argmax / sum / sum-of-squares over the K
 // correlations of the step, then one merge into the running maps.
-template <int WS, int MODE, int TWM>
-__global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
+// Block -> (tile row p, row group sub): with RB < TR rows per block, the TR / RB blocks of
+// one tile row get ids 8 apart so they run on the same XCD (workgroups are dispatched to
+// the 8 XCDs round robin) and share its L2 for the 64-byte tiles they split.
+template <int RB>
+__device__ __forceinline__ void rowpass_block(int& p, int& sub) {
+  constexpr int NS = TR / RB;
+  const int b = blockIdx.x;
+  if constexpr (NS == 1) {
+    p = b;
+    sub = 0;
+  } else {
+    const int rest = b >> 3;
+    sub = rest % NS;
+    p = (rest / NS) * 8 + (b & 7);
+  }
+}
+
+// (launch bounds: at least 4 waves per SIMD, i.e. at most 128 VGPRs, so that with RB < TR
+// several blocks share a CU)
+template <int WS, int MODE, int TWM, int RB>
+__global__ __launch_bounds__(RB*(WS / 2) / V, 4) void k_rowpass(const float2* __restrict__ Gb, int HH, int G, int K,
                                                             int nb, int nd, int O, int o0,
                                                             const float2* __restrict__ tw, float* __restrict__ mip,
                                                             int* __restrict__ bidx, float* __restrict__ shi,
@@ -777,10 +819,11 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
                                                             float* __restrict__ qlo) {
   using C = RowCfg<WS>;
   constexpr int T = C::T;
-  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE>::floats + (TWM ? RowTw<WS>::floats : 0)];
+  __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE, RB>::floats + (TWM ? RowTw<WS>::floats : 0)];
   const int r = threadIdx.x / T, t = threadIdx.x % T;
-  const int p = blockIdx.x;  // row-tile index
-  const int y = p * TR + r;
+  int p, sub;  // tile row, row group within it
+  rowpass_block<RB>(p, sub);
+  const int y = p * TR + sub * RB + r;
   (void)G;
   // running max / sum / sum of squares per pixel; the correlation index of the max is
   // kept as a 16-bit step-local index, two pixels per register (K < 65536)
@@ -794,7 +837,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass(const float2* __res
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
-  rowpass_loop<WS, MODE, TWM>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, MODE, TWM, RB>(Gb, HH, K, p, sub * RB, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -962,7 +1005,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  rowpass_loop<WS, 3, 1>(Gb, HH, K, p, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, 3, 1, TR>(Gb, HH, K, p, 0, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -1129,6 +1172,7 @@ struct tdtm_match_plan {
   int opt_rowpass_staging = 3;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat,
                                 // 3: synchronous de-interleaved with batched loads
   int opt_rowpass_twiddles = 1; // 0: twiddles from the global table, 1: LDS copy (staging 0 / 3 only)
+  int opt_rowpass_rows = 1;     // rows of a tile row per row-pass block: 4, 2 or 1 (staging 0 / 3)
   int opt_phase_timing = 0;     // record HIP events between the phases of every step
   // per-step events at the phase boundaries (project | rowfwd | colpass | rowpass), read and
   // released by tdtm_match_plan_phase_times
@@ -1220,7 +1264,8 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   constexpr bool FOUR = UseFour<HH>::value;
   const int cols = (FOUR && p->opt_colpass_columns == 1) ? 1 : CW;
   const int nt = (FOUR && p->opt_colpass_pair) ? 2 : 1;
-  const int nblk_x = p->G * (CW / cols);
+  // one column per block: grid padded to groups of 16 (see k_colpass)
+  const int nblk_x = cols == 1 ? (2 * p->G + 15) / 16 * 16 : p->G;
   int nchunk = p->opt_colpass_chunks > 0 ? p->opt_colpass_chunks : (2048 + nblk_x - 1) / nblk_x;
   if (nchunk > K) nchunk = K;
   if (nchunk < 1) nchunk = 1;
@@ -1267,11 +1312,21 @@ static void launch_colpass(tdtm_match_plan* p, int K, int ystore, hipStream_t st
   else launch_colpass_n<HH, 8>(p, K, ystore, st);
 }
 
+template <int WS, int MODE, int TWM, int RB>
+static void launch_rowpass_rb(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
+                              float* slo, float* qhi, float* qlo, hipStream_t st) {
+  k_rowpass<WS, MODE, TWM, RB><<<dim3((p->H / TR) * (TR / RB)), RB * (WS / 2) / V, 0, st>>>(
+      p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi, qlo);
+}
+
 template <int WS, int MODE, int TWM>
 static void launch_rowpass_mt(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                               float* slo, float* qhi, float* qlo, hipStream_t st) {
-  k_rowpass<WS, MODE, TWM><<<dim3(p->H / TR), TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, nb, p->D, O, o0,
-                                                                         p->tw, mip, idx, shi, slo, qhi, qlo);
+  if constexpr (MODE == 0 || MODE == 3) {
+    if (p->opt_rowpass_rows == 1) return launch_rowpass_rb<WS, MODE, TWM, 1>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+    if (p->opt_rowpass_rows == 2) return launch_rowpass_rb<WS, MODE, TWM, 2>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+  }
+  launch_rowpass_rb<WS, MODE, TWM, TR>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
 }
 
 template <int WS>
@@ -1415,7 +1470,14 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     if (value < 0 || value > 3) return fail("rowpass_staging must be 0, 1, 2 or 3");
     if ((value == 1 || value == 2) && p->opt_rowpass_twiddles)
       return fail("rowpass_staging 1 / 2 leave no LDS for rowpass_twiddles = 1");
+    if ((value == 1 || value == 2) && p->opt_rowpass_rows != 4)
+      return fail("rowpass_staging 1 / 2 need rowpass_rows = 4");
     p->opt_rowpass_staging = value;
+  } else if (k == "rowpass_rows") {
+    if (value != 1 && value != 2 && value != 4) return fail("rowpass_rows must be 1, 2 or 4");
+    if (value != 4 && (p->opt_rowpass_staging == 1 || p->opt_rowpass_staging == 2))
+      return fail("rowpass_rows < 4 needs rowpass_staging 0 or 3");
+    p->opt_rowpass_rows = value;
   } else if (k == "rowpass_twiddles") {
     if (value < 0 || value > 1) return fail("rowpass_twiddles must be 0 or 1");
     if (value && (p->opt_rowpass_staging == 1 || p->opt_rowpass_staging == 2))

@@ -31,6 +31,8 @@ CANDIDATES: list[dict[str, int]] = [
     {"colpass_columns": 1, "colpass_pair": 0},
     {"rowpass_staging": 0},
     {"rowpass_twiddles": 0},
+    {"rowpass_rows": 2},
+    {"rowpass_rows": 4},
 ]
 
 _OPTION_DEFAULTS = {
@@ -40,6 +42,7 @@ _OPTION_DEFAULTS = {
     "colpass_twiddles": 1,
     "rowpass_twiddles": 1,
     "rowpass_staging": 3,
+    "rowpass_rows": 1,
 }
 
 
@@ -52,13 +55,17 @@ def apply_options(plan, options: dict[str, int]) -> None:
     """Reset every tunable option to its default, then apply ``options`` (row-pass
     twiddles in LDS are switched off for the staging modes that leave no room)."""
     plan.set_option("rowpass_twiddles", 0)
-    for k in ("colpass_columns", "colpass_pair", "colpass_chunks", "colpass_twiddles", "rowpass_staging"):
+    plan.set_option("rowpass_staging", _OPTION_DEFAULTS["rowpass_staging"])
+    for k in ("colpass_columns", "colpass_pair", "colpass_chunks", "colpass_twiddles", "rowpass_rows"):
         plan.set_option(k, _OPTION_DEFAULTS[k])
     want = dict(options)
     tw = want.pop("rowpass_twiddles", None)
-    staging = want.get("rowpass_staging", _OPTION_DEFAULTS["rowpass_staging"])
+    staging = want.pop("rowpass_staging", _OPTION_DEFAULTS["rowpass_staging"])
+    if staging in (1, 2):
+        want["rowpass_rows"] = 4  # asynchronous staging copies whole tile rows
     for k, v in want.items():
         plan.set_option(k, v)
+    plan.set_option("rowpass_staging", staging)
     if tw is None:
         tw = 0 if staging in (1, 2) else _OPTION_DEFAULTS["rowpass_twiddles"]
     plan.set_option("rowpass_twiddles", tw)

@@ -152,9 +152,10 @@ def test_match_peak_list_identical(cuda_device):
 
 @pytest.mark.parametrize("opts", [
     {"colpass_columns": 1}, {"colpass_pair": 1}, {"rowpass_staging": 0}, {"rowpass_twiddles": 0},
-    {"rowpass_twiddles": 0, "rowpass_staging": 0}, {"rowpass_twiddles": 0, "rowpass_staging": 1},
-    {"rowpass_twiddles": 0, "rowpass_staging": 2}, {"colpass_chunks": 3}, {"colpass_columns": 1, "colpass_pair": 0},
+    {"rowpass_twiddles": 0, "rowpass_staging": 0}, {"rowpass_rows": 4, "rowpass_twiddles": 0, "rowpass_staging": 1},
+    {"rowpass_rows": 4, "rowpass_twiddles": 0, "rowpass_staging": 2}, {"rowpass_rows": 4}, {"colpass_chunks": 3}, {"colpass_columns": 1, "colpass_pair": 0},
     {"colpass_twiddles": 0}, {"colpass_twiddles": 0, "colpass_pair": 1}, {"colpass_columns": 1, "colpass_chunks": 5},
+    {"rowpass_rows": 2}, {"rowpass_rows": 1}, {"rowpass_rows": 1, "rowpass_twiddles": 0, "rowpass_staging": 0},
     {"phase_timing": 1},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
@@ -187,12 +188,17 @@ def test_unknown_launch_option_raises(cuda_device):
         plan.set_option("colpass_columns", 3)
     plan.set_option("rowpass_staging", 0)
     plan.set_option("rowpass_twiddles", 1)
+    plan.set_option("rowpass_rows", 4)
     with pytest.raises(_lib.TdtmError, match="LDS"):
         plan.set_option("rowpass_staging", 1)
     plan.set_option("rowpass_twiddles", 0)
     plan.set_option("rowpass_staging", 2)
     with pytest.raises(_lib.TdtmError, match="rowpass_staging"):
         plan.set_option("rowpass_twiddles", 1)
+    with pytest.raises(_lib.TdtmError, match="rowpass_staging"):
+        plan.set_option("rowpass_rows", 2)
+    with pytest.raises(_lib.TdtmError, match="rowpass_rows"):
+        plan.set_option("rowpass_rows", 3)
 
 
 def test_phase_times(cuda_device):
