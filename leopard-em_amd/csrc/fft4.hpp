@@ -86,6 +86,93 @@ __device__ __forceinline__ void twiddle_mul_n(float2 (&a)[NT][R], TWT tw, int k,
   }
 }
 
+// Lane-ordered twiddle bases of the four-step transform of size N, for a copy in LDS.
+// Each stage needs W^{k}, W^{2k}, W^{4k} (radix 8; fewer for radix NW < 8) for its own
+// per-lane k, so they are stored as arrays indexed by k -- lanes read consecutive
+// entries (or, in stage 3, broadcast 8 of them), free of bank conflicts:
+//   stage 1  A1[e][t] = W_N^{2^e t}   (e < 3, t < T)
+//   stage 2  A2[e][l] = W_S^{2^e l}   (e < log2 NW, l < 64)
+//   stage 3  A3[e][p] = W_64^{2^e p}  (e < 3, p < 8)
+template <int N>
+struct FsTwLayout {
+  static constexpr int T = FS<N>::T, NW = FS<N>::NW, S = FS<N>::S;
+  static constexpr int LNW = NW == 1 ? 0 : NW == 2 ? 1 : NW == 4 ? 2 : 3;
+  static constexpr int O1 = 0, O2 = 3 * T, O3 = O2 + LNW * 64;
+  static constexpr int entries = O3 + 3 * 8;
+  static constexpr int floats = 2 * entries;
+  // table index of entry j (every index is a multiple of TW_N / N below TW_N / 2)
+  __device__ static int index(int j) {
+    if (j < O2) return ((j % T) << (j / T)) * (TW_N / N);
+    if (j < O3) return (((j - O2) % 64) << ((j - O2) / 64)) * (TW_N / S);
+    return (((j - O3) % 8) << ((j - O3) / 8)) * (TW_N / 64);
+  }
+};
+
+template <int N>
+struct FsTw {
+  const float2* p;  // LDS copy in FsTwLayout<N> order
+};
+
+template <int N>
+__device__ __forceinline__ FsTw<N> opaque_offset(FsTw<N> t) {
+  return FsTw<N>{opaque_offset(t.p)};
+}
+
+// Fill an FsTwLayout<N> copy from the global table (all threads of the block; the caller
+// synchronises before use).
+template <int N>
+__device__ __forceinline__ void fs_tw_fill(float2* dst, const float2* __restrict__ tw, int tid, int nthreads) {
+  for (int j = tid; j < FsTwLayout<N>::entries; j += nthreads) dst[j] = tw[FsTwLayout<N>::index(j)];
+}
+
+template <class TWT>
+struct IsFsTw {
+  static constexpr bool value = false;
+};
+template <int N>
+struct IsFsTw<FsTw<N>> {
+  static constexpr bool value = true;
+};
+
+// Twiddles of four-step stage STAGE (1: W_N^{j t}, 2: W_S^{j l}, 3: W_64^{j p}) for lane
+// index k: w[j] for j < R, from a lane-ordered LDS copy or any indexable table.
+template <int STAGE, int N, int R, bool INV, class TWT>
+__device__ __forceinline__ void fs_tw(float2 (&w)[R], TWT tw, int k) {
+  using L = FsTwLayout<N>;
+  constexpr int step = STAGE == 1 ? TW_N / N : STAGE == 2 ? TW_N / L::S : TW_N / 64;
+  if constexpr (IsFsTw<TWT>::value) {
+    static_assert(STAGE != 2 || (R == L::NW), "stage 2 is radix NW");
+    constexpr int off = STAGE == 1 ? L::O1 : STAGE == 2 ? L::O2 : L::O3;
+    constexpr int cnt = STAGE == 1 ? L::T : STAGE == 2 ? 64 : 8;
+    auto ld = [&](int e) {
+      const float2 v = tw.p[off + e * cnt + k];
+      return INV ? c_conj(v) : v;
+    };
+    w[0] = make_float2(1.f, 0.f);
+    if constexpr (R >= 2) w[1] = ld(0);
+    if constexpr (R >= 4) {
+      w[2] = ld(1);
+      w[3] = c_mul(w[1], w[2]);
+    }
+    if constexpr (R >= 8) {
+      w[4] = ld(2);
+      w[5] = c_mul(w[1], w[4]);
+      w[6] = c_mul(w[2], w[4]);
+      w[7] = c_mul(w[3], w[4]);
+    }
+  } else {
+    twiddles<R, INV>(w, tw, k, step);
+  }
+}
+
+template <int R, int NT>
+__device__ __forceinline__ void apply_tw_n(float2 (&a)[NT][R], const float2 (&w)[R]) {
+#pragma unroll
+  for (int n = 0; n < NT; ++n)
+#pragma unroll
+    for (int j = 1; j < R; ++j) a[n][j] = c_mul(a[n][j], w[j]);
+}
+
 template <int N, bool INV, int NZJ, int NT, class TWT>
 __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
                                               int t, TWT tw) {
@@ -96,7 +183,11 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   // stage 1: radix-8 over i, twiddle W_N^{k1 t}
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft_sparse<8, NZJ, INV>(v[n], twp);
-  twiddle_mul_n<8, INV, NT>(v, twp, t, TW_N / N);
+  {
+    float2 w1[8];
+    fs_tw<1, N, 8, INV>(w1, twp, t);
+    apply_tw_n<8, NT>(v, w1);
+  }
   // E1: row k1, column t
 #pragma unroll
   for (int n = 0; n < NT; ++n)
@@ -119,7 +210,7 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   // stage 2: radix-NW over j, twiddle W_S^{m1 l}
   {
     float2 wt[NW];
-    twiddles<NW, INV>(wt, twp, l, TW_N / F::S);
+    fs_tw<2, N, NW, INV>(wt, twp, l);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
@@ -158,7 +249,11 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   // stage 3: radix-8 over q, twiddle W_64^{p r}
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft<8, INV>(b[n]);
-  twiddle_mul_n<8, INV, NT>(b, twp, p, TW_N / 64);
+  {
+    float2 w3[8];
+    fs_tw<3, N, 8, INV>(w3, twp, p);
+    apply_tw_n<8, NT>(b, w3);
+  }
   // E3 (wave-local): [grp][r][p]
   wave_lds_order();
 #pragma unroll
@@ -220,7 +315,11 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
     for (int r = 0; r < 8; ++r) b[n][r] = make_float2(wr[grp * 72 + r * 9 + p], wi[grp * 72 + r * 9 + p]);
   }
   // stage 3^T: twiddle W_64^{p r}, radix-8 over r -> q
-  twiddle_mul_n<8, INV, NT>(b, twp, p, TW_N / 64);
+  {
+    float2 w3[8];
+    fs_tw<3, N, 8, INV>(w3, twp, p);
+    apply_tw_n<8, NT>(b, w3);
+  }
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft<8, INV>(b[n]);
   // E2^T: lane (grp, p) holds column p + 8 q of row grp
@@ -246,7 +345,7 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
   // stage 2^T: twiddle W_S^{m1 l}, radix-NW over m1 -> j
   {
     float2 wt[NW];
-    twiddles<NW, INV>(wt, twp, l, TW_N / F::S);
+    fs_tw<2, N, NW, INV>(wt, twp, l);
 #pragma unroll
     for (int n = 0; n < NT; ++n)
 #pragma unroll
@@ -280,7 +379,11 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
     for (int k1 = 0; k1 < 8; ++k1) v[n][k1] = make_float2(re[n][k1 * RS + t], im[n][k1 * RS + t]);
   lds_barrier();
   // stage 1^T: twiddle W_N^{k1 t}, radix-8 over k1 -> i
-  twiddle_mul_n<8, INV, NT>(v, twp, t, TW_N / N);
+  {
+    float2 w1[8];
+    fs_tw<1, N, 8, INV>(w1, twp, t);
+    apply_tw_n<8, NT>(v, w1);
+  }
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
 }

@@ -434,9 +434,9 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   static_assert(CWB == CW || (CWB == 1 && FOUR), "one column per block needs the four-step path");
   static_assert(NT == 1 || FOUR, "paired correlations need the four-step path");
   static_assert(TWM == 0 || FOUR, "the LDS twiddle table serves the four-step path");
-  // LDS copy: entries j * (TW_N / HH) for j < HH / 2 -- the four-step transforms of size
-  // HH read only multiples of TW_N / HH below TW_N / 2
-  __shared__ float lds[CWB * NT * 2 * LF + (TWM ? HH : 0)];
+  // TWM == 1: lane-ordered copy of the four-step twiddle bases (FsTwLayout) after the
+  // transforms' work areas
+  __shared__ float lds[CWB * NT * 2 * LF + (TWM ? FsTwLayout<HF>::floats : 0)];
   // block -> (tile column g, column c): with one column per block, the two columns of a
   // tile column get block ids 8 apart, so they run on the same XCD (workgroups go to the
   // 8 XCDs round robin) and their halves of every 64-byte tile meet in its L2; the grid
@@ -452,15 +452,13 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
     if (g >= G) return;
   }
   float2* ltw = reinterpret_cast<float2*>(lds + CWB * NT * 2 * LF);
-  constexpr int SH = tw_shift(HH);
-  static_assert((HH << SH) == TW_N, "column length must divide the twiddle table");
   if constexpr (TWM == 1) {
-    for (int j = threadIdx.x; j < HH / 2; j += CWB * T) ltw[j] = tw_global[j << SH];
+    fs_tw_fill<HF>(ltw, tw_global, threadIdx.x, CWB * T);
     __syncthreads();
   }
   const auto tw = [&]() {
     if constexpr (TWM == 1)
-      return TwLds<SH>{ltw};
+      return FsTw<HF>{ltw};
     else
       return tw_global;
   }();
@@ -695,13 +693,34 @@ struct RowLds {
 // log2(TW_N / WS): a W-point row pass reads only table entries that are multiples of
 // TW_N / W and below TW_N / 2 (pre-twiddles m < W/2, transform twiddles of the W/2-point
 // four-step), so its LDS twiddle copy holds W / 2 entries.
+// With the four-step transform (M >= 512) the copy is lane-ordered instead: the
+// FsTwLayout<M> stage bases followed by the pre-twiddle bases PB[t] (see rowpass_loop).
 template <int WS>
 struct RowTw {
   static constexpr int SH = tw_shift(WS);
   static_assert((WS << SH) == TW_N, "row length must divide the twiddle table");
   static constexpr int entries = WS / 2;
-  static constexpr int floats = 2 * entries;
+  static constexpr bool FOUR = RowCfg<WS>::FOUR;
+  static constexpr int MF = RowCfg<WS>::MF;
+  static constexpr int floats = FOUR ? FsTwLayout<MF>::floats + 2 * RowCfg<WS>::T : 2 * entries;
 };
+
+// W_16^i = e^{-2 pi i i / 16}: the pre-twiddle of slot i of a four-step row thread is
+// W_W^{fs_index(t, i)} = PB[t] W_16^i with PB[t] = W_W^{fs_index(t, 0)}, because
+// fs_index(t, i) = fs_index(t, 0) + (M / 8) i and W_W^{M/8} = W_16.
+__device__ __forceinline__ float2 w16_times(float2 a, int i) {
+  constexpr float C1 = 0.92387953251128675613f, S1 = 0.38268343236508977173f, H = 0.70710678118654752440f;
+  switch (i) {
+    case 0: return a;
+    case 1: return c_mul(a, make_float2(C1, -S1));
+    case 2: return c_mul(a, make_float2(H, -H));
+    case 3: return c_mul(a, make_float2(S1, -C1));
+    case 4: return make_float2(a.y, -a.x);
+    case 5: return c_mul(a, make_float2(-S1, -C1));
+    case 6: return c_mul(a, make_float2(-H, -H));
+    default: return c_mul(a, make_float2(-C1, -S1));
+  }
+}
 
 template <int WS, int MODE, int TWM, int RB, class Epi>
 __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p, int r0,
@@ -710,15 +729,24 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
   constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
   constexpr bool ASYNC = MODE == 1;
   const int r = threadIdx.x / T, t = threadIdx.x % T;
-  // twiddle source: the global table, or (TWM == 1) a copy of its every (TW_N / WS)-th
-  // entry placed after the staging / work area
+  // twiddle source: the global table, or (TWM == 1) an LDS copy after the staging / work
+  // area -- lane-ordered stage and pre-twiddle bases for the four-step transform, every
+  // (TW_N / WS)-th table entry otherwise
   float2* ltw = reinterpret_cast<float2*>(lds + RowLds<WS, MODE, RB>::floats);
+  float2* lpb = ltw + (C::FOUR ? FsTwLayout<MF>::entries : 0);  // PB[t] (four-step)
   if constexpr (TWM == 1) {
-    for (int j = threadIdx.x; j < RowTw<WS>::entries; j += C::BLOCK) ltw[j] = tw_global[j << RowTw<WS>::SH];
+    if constexpr (C::FOUR) {
+      fs_tw_fill<MF>(ltw, tw_global, threadIdx.x, C::BLOCK);
+      for (int j = threadIdx.x; j < T; j += C::BLOCK) lpb[j] = tw_global[fs_index<MF>(j, 0) * (TW_N / WS)];
+    } else {
+      for (int j = threadIdx.x; j < RowTw<WS>::entries; j += C::BLOCK) ltw[j] = tw_global[j << RowTw<WS>::SH];
+    }
     __syncthreads();
   }
   const auto tw = [&]() {
-    if constexpr (TWM == 1)
+    if constexpr (TWM == 1 && C::FOUR)
+      return FsTw<MF>{ltw};
+    else if constexpr (TWM == 1)
       return TwLds<RowTw<WS>::SH>{ltw};
     else
       return tw_global;
@@ -754,6 +782,13 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
     const float* sim = sre + LF;
     float2 v[V];
     const auto twp = opaque_offset(tw);  // reload the pre-twiddles per correlation (no hoisting)
+    float2 pb = make_float2(1.f, 0.f);   // four-step: PB[t], the pre-twiddle base
+    if constexpr (C::FOUR) {
+      if constexpr (TWM == 1)
+        pb = opaque_offset(static_cast<const float2*>(lpb))[t];
+      else
+        pb = twp[fs_index<MF>(t, 0) * (TW_N / WS)];
+    }
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       int mm = t + i * T;
@@ -772,7 +807,11 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
         A.y = 0.f;
         Bc.y = 0.f;
       }
-      const float2 w = c_conj(twp[mm * (TW_N / WS)]);
+      float2 w;
+      if constexpr (C::FOUR)
+        w = c_conj(w16_times(pb, i));
+      else
+        w = c_conj(twp[mm * (TW_N / WS)]);
       const float2 dsum = c_add(A, Bc), od = c_mul(w, c_sub(A, Bc));
       v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);
     }
@@ -1273,8 +1312,9 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
   nchunk = (K + kchunk - 1) / kchunk;
   dim3 grid(nblk_x, nchunk);
-  // the four-step LDS twiddle copy (HH / 2 entries) when it fits next to the transforms
+  // the four-step LDS twiddle copy when it fits next to the transforms
   constexpr int LFc = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
+  constexpr int TWF = FsTwLayout<(FOUR ? HH : 512)>::floats;
   const bool full = ystore == HH;
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
 #define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                                      \
@@ -1282,7 +1322,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
                                                                                ystore, p->tw, p->Gb)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
-    constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + HH) * 4 <= TDTM_LDS_BYTES;      \
+    constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
     if (ltw && fits_) {                                                                   \
       if (full) TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), true);                         \
       else TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), false);                             \
