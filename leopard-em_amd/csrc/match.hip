@@ -302,7 +302,14 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
   __shared__ float lds[2 * LF];
   float* lre = lds;
   float* lim = lds + LF;
-  const int y = blockIdx.x;
+  // rows 8m .. 8m+7 share the 128-byte lines of Rb (16 bytes each per tile column): with
+  // hs a multiple of 64 their blocks get ids 8 apart, so they run on the same XCD
+  // (workgroups go to the 8 XCDs round robin) and complete the lines in its L2
+  int y = blockIdx.x;
+  if (hs % 64 == 0) {
+    const int b = blockIdx.x;
+    y = ((((b >> 6) << 3) + (b & 7)) << 3) + ((b >> 3) & 7);
+  }
   const int k = blockIdx.y;
   const int t = threadIdx.x;
   const float2 s = ae[k];
@@ -377,9 +384,9 @@ __device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // stores.
 template <int HH, int NZJ, int NT, int NX, bool FULL, class TWT>
 __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
-                                             int hs, int G, int g, int c, int t, int k, int klast, int ystore, bool dc,
-                                             const float2 (&xi)[V], float* const (&re)[NT], float* const (&im)[NT],
-                                             TWT tw, float2* __restrict__ Gb) {
+                                             int hs, int G, int Gs, int g, int c, int t, int k, int klast, int ystore,
+                                             bool dc, const float2 (&xi)[V], float* const (&re)[NT],
+                                             float* const (&im)[NT], TWT tw, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr int NL = NZJ < V ? NZJ : V;
   float2 v[NT][V];
@@ -399,14 +406,15 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
     if (dc) v[n][0] = make_float2(0.f, 0.f);
   }
   fs_perm2nat_n<HH, true, NT>(v, re, im, t, tw);
-  // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR
+  // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
+  // tile rows are Gs tiles apart
   constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float2* gout = Gb + (long)(k + n) * HH * G * CW + ((long)(t / TR) * G + g) * TR * CW + (t % TR) * CW + c;
+    float2* gout = Gb + (long)(k + n) * HH * Gs * CW + ((long)(t / TR) * Gs + g) * TR * CW + (t % TR) * CW + c;
 #pragma unroll
     for (int i = 0; i < V; ++i)
-      if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * G] = v[n][i];
+      if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * Gs] = v[n][i];
   }
 }
 
@@ -424,7 +432,7 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // four-step transforms of size HH use; FULL = every row is stored (ystore == HH).
 template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
-                                                         int hs, int G, int K, int kchunk, int ystore,
+                                                         int hs, int G, int Gs, int K, int kchunk, int ystore,
                                                          const float2* __restrict__ tw_global, float2* __restrict__ Gb) {
   constexpr int T = HH / V;
   constexpr bool FOUR = UseFour<HH>::value;
@@ -437,18 +445,20 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   // TWM == 1: lane-ordered copy of the four-step twiddle bases (FsTwLayout) after the
   // transforms' work areas
   __shared__ float lds[CWB * NT * 2 * LF + (TWM ? FsTwLayout<HF>::floats : 0)];
-  // block -> (tile column g, column c): with one column per block, the two columns of a
-  // tile column get block ids 8 apart, so they run on the same XCD (workgroups go to the
-  // 8 XCDs round robin) and their halves of every 64-byte tile meet in its L2; the grid
-  // is padded to whole groups of 16 and the blocks past the last tile column exit
-  int g, c;
-  if constexpr (CWB == CW) {
-    g = blockIdx.x;
-    c = 0;
-  } else {
-    const int b = blockIdx.x, rest = b >> 3;
-    c = rest & 1;
-    g = (rest >> 1) * 8 + (b & 7);
+  // block -> (tile column g, column c): the blocks that write the same 128-byte lines of
+  // Gb -- tile columns 2m and 2m+1 (and both columns of each with one column per block)
+  // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
+  // robin) and complete the lines in its L2; the grid is padded to whole groups and the
+  // blocks past the last tile column exit
+  int g, c = 0;
+  {
+    const int b = blockIdx.x, q = b >> 3;
+    if constexpr (CWB == CW) {
+      g = 2 * ((q >> 1) * 8 + (b & 7)) + (q & 1);
+    } else {
+      c = q & 1;
+      g = 2 * ((q >> 2) * 8 + (b & 7)) + ((q >> 1) & 1);
+    }
     if (g >= G) return;
   }
   float2* ltw = reinterpret_cast<float2*>(lds + CWB * NT * 2 * LF);
@@ -492,26 +502,26 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       // wait for the inputs then leaves the stores in flight)
       if (k + 1 < k1) {
         colpass_load<HF, NZJ, 2>(in, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
         k += 2;
       }
       for (; k + 1 < k1; k += 2)
-        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
       // a single last correlation: its input was prefetched into in[0], unless the chunk
       // holds only this one
       if (k < k1) {
         if (k == k0) colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 0, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 0, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
       }
     } else {
       const int klast = k1 - 1;
       if (k < k1) {  // peeled first iteration (see the paired loop)
         colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
         ++k;
       }
       for (; k < k1; ++k)
-        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
     }
   } else {
     float* lre = base;
@@ -529,11 +539,11 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       for (int i = 0; i < V; ++i) v[i] = c_mulc(xi[i], v[i]);
       if (dc) v[0] = make_float2(0.f, 0.f);
       fft_wg<HH, V, true>(v, lre, lim, t, tw);
-      float2* gout = Gb + (long)k * HH * G * CW;
+      float2* gout = Gb + (long)k * HH * Gs * CW;
 #pragma unroll
       for (int i = 0; i < V; ++i) {
         const int yy = t + i * T;
-        if (FULL || yy < ystore) gout[(((long)(yy / TR) * G + g) * TR + (yy % TR)) * CW + c] = v[i];
+        if (FULL || yy < ystore) gout[(((long)(yy / TR) * Gs + g) * TR + (yy % TR)) * CW + c] = v[i];
       }
     }
   }
@@ -559,7 +569,8 @@ struct RowCfg {
   static constexpr int MF = FOUR ? M : 512;
   static constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;  // per component per row
   static constexpr int G = (M + 1 + CW - 1) / CW;                          // tile columns
-  static constexpr int TILEF = G * TR * CW;                                // complex per tile row in HBM
+  static constexpr int GS = G + (G & 1);                                   // tile stride (even)
+  static constexpr int TILEF = GS * TR * CW;                               // complex per tile row in HBM
   static constexpr int TILE = G * RB * CW;                                 // complex staged per block
   static constexpr int RSTR = 2 * LF + 16;                                 // floats per row region
   static constexpr int REGION = RB * RSTR;                                 // floats, FFT work area
@@ -1203,6 +1214,7 @@ using namespace tdtm;
 
 struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
+  int Gs;  // tile-column stride of the intermediate (G rounded up to even: 128-byte lines)
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
@@ -1304,7 +1316,8 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   const int cols = (FOUR && p->opt_colpass_columns == 1) ? 1 : CW;
   const int nt = (FOUR && p->opt_colpass_pair) ? 2 : 1;
   // one column per block: grid padded to groups of 16 (see k_colpass)
-  const int nblk_x = cols == 1 ? (2 * p->G + 15) / 16 * 16 : p->G;
+  const int npair8 = ((p->G + 1) / 2 + 7) / 8;  // groups of 8 tile-column pairs
+  const int nblk_x = cols == 1 ? 32 * npair8 : 16 * npair8;
   int nchunk = p->opt_colpass_chunks > 0 ? p->opt_colpass_chunks : (2048 + nblk_x - 1) / nblk_x;
   if (nchunk > K) nchunk = K;
   if (nchunk < 1) nchunk = 1;
@@ -1318,8 +1331,8 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   const bool full = ystore == HH;
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
 #define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                                      \
-  k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, K, kchunk, \
-                                                                               ystore, p->tw, p->Gb)
+  k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, p->Gs, K, \
+                                                                               kchunk, ystore, p->tw, p->Gb)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
@@ -1432,6 +1445,7 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   p->Wr = W / 2 + 1;
   p->wr = h / 2 + 1;
   p->G = (p->Wr + CW - 1) / CW;
+  p->Gs = p->G + (p->G & 1);
   const size_t K = p->K;
   int rc = 0;
   rc |= dalloc(&p->tw, TW_N, p->bytes);
@@ -1443,7 +1457,7 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   rc |= rc ? 0 : dalloc(&p->part, K * h * 3, p->bytes);
   rc |= rc ? 0 : dalloc(&p->ae, K, p->bytes);
   rc |= rc ? 0 : dalloc(&p->Rb, K * p->G * h * CW, p->bytes);
-  rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->G * CW, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->Gs * CW, p->bytes);
   rc |= rc ? 0 : dalloc(&p->ctfbuf, (size_t)C * D * h * p->wr, p->bytes);
   rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR, p->bytes);
   if (rc) {
