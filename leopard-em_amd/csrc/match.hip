@@ -1215,6 +1215,7 @@ using namespace tdtm;
 struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
   int Gs;  // tile-column stride of the intermediate (G rounded up to even: 128-byte lines)
+  int num_cu = 256;  // compute units of the plan's device
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
@@ -1318,21 +1319,14 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   // one column per block: grid padded to groups of 16 (see k_colpass)
   const int npair8 = ((p->G + 1) / 2 + 7) / 8;  // groups of 8 tile-column pairs
   const int nblk_x = cols == 1 ? 32 * npair8 : 16 * npair8;
-  int nchunk = p->opt_colpass_chunks > 0 ? p->opt_colpass_chunks : (2048 + nblk_x - 1) / nblk_x;
-  if (nchunk > K) nchunk = K;
-  if (nchunk < 1) nchunk = 1;
-  int kchunk = (K + nchunk - 1) / nchunk;
-  if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
-  nchunk = (K + kchunk - 1) / kchunk;
-  dim3 grid(nblk_x, nchunk);
   // the four-step LDS twiddle copy when it fits next to the transforms
   constexpr int LFc = FOUR ? FS<(FOUR ? HH : 512)>::floats : LdsSize<HH>::floats;
   constexpr int TWF = FsTwLayout<(FOUR ? HH : 512)>::floats;
   const bool full = ystore == HH;
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
-#define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                                      \
-  k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_><<<grid, CWB_ * (HH / V), 0, st>>>(p->Rb, p->Xt, p->h, p->G, p->Gs, K, \
-                                                                               kchunk, ystore, p->tw, p->Gb)
+  using Kern = void (*)(const float2*, const float2*, int, int, int, int, int, int, const float2*, float2*);
+  Kern kern = nullptr;
+#define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_) kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_>
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
@@ -1354,6 +1348,37 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   }
 #undef TDTM_COLPASS
 #undef TDTM_COLPASS_T
+  const int threads = cols * (HH / V);
+  // correlation chunks per column group: the split that minimises (rounds of resident
+  // blocks) x (correlations per block + 2 for the block's set-up: image column, twiddle
+  // table, first load), i.e. the last round of blocks is as full as possible without
+  // many short blocks; resident blocks per CU from the runtime's occupancy calculator
+  int nchunk = p->opt_colpass_chunks;
+  if (nchunk <= 0) {
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    const long slots = (long)per_cu * p->num_cu;
+    long best = -1;
+    for (int n = 1; n <= 16 && n <= K; ++n) {
+      int kc = (K + n - 1) / n;
+      if (nt == 2 && (kc & 1) && kc < K) ++kc;
+      const int nn = (K + kc - 1) / kc;
+      const long rounds = ((long)nblk_x * nn + slots - 1) / slots;
+      const long cost = rounds * (kc + 2);
+      if (best < 0 || cost < best) {
+        best = cost;
+        nchunk = nn;
+      }
+    }
+  }
+  if (nchunk > K) nchunk = K;
+  if (nchunk < 1) nchunk = 1;
+  int kchunk = (K + nchunk - 1) / nchunk;
+  if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
+  nchunk = (K + kchunk - 1) / kchunk;
+  hipLaunchKernelGGL(kern, dim3(nblk_x, nchunk), dim3(threads), 0, st, (const float2*)p->Rb, (const float2*)p->Xt,
+                     p->h, p->G, p->Gs, K, kchunk, ystore, (const float2*)p->tw, p->Gb);
 }
 
 template <int HH>
@@ -1435,6 +1460,12 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   if (h > H || h > W) return fail("template larger than image");
   if (B < 1 || D < 1 || C < 1) return fail("batch, defocus and pixel counts must be >= 1");
   auto* p = new tdtm_match_plan();
+  {
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && ncu > 0)
+      p->num_cu = ncu;
+  }
   p->H = H;
   p->W = W;
   p->h = h;
