@@ -173,12 +173,36 @@ __device__ __forceinline__ void apply_tw_n(float2 (&a)[NT][R], const float2 (&w)
     for (int j = 1; j < R; ++j) a[n][j] = c_mul(a[n][j], w[j]);
 }
 
-template <int N, bool INV, int NZJ, int NT, class TWT>
+// LDS element access of transform n: split real / imaginary float arrays re[n] / im[n],
+// or (IL) one interleaved float2 array starting at re[n] (im unused) -- same indices, one
+// 8-byte access per element instead of two 4-byte ones.
+template <bool IL, int NT>
+struct FsLds {
+  float* const (&re)[NT];
+  float* const (&im)[NT];
+  __device__ __forceinline__ void st(int n, int i, float2 v) const {
+    if constexpr (IL) {
+      reinterpret_cast<float2*>(re[n])[i] = v;
+    } else {
+      re[n][i] = v.x;
+      im[n][i] = v.y;
+    }
+  }
+  __device__ __forceinline__ float2 ld(int n, int i) const {
+    if constexpr (IL)
+      return reinterpret_cast<const float2*>(re[n])[i];
+    else
+      return make_float2(re[n][i], im[n][i]);
+  }
+};
+
+template <int N, bool INV, int NZJ, int NT, bool IL = false, class TWT>
 __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
                                               int t, TWT tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
+  const FsLds<IL, NT> L{re, im};
   const auto twp = opaque_offset(tw);
   // stage 1: radix-8 over i, twiddle W_N^{k1 t}
 #pragma unroll
@@ -193,8 +217,7 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   for (int n = 0; n < NT; ++n)
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) {
-      re[n][k1 * RS + t] = v[n][k1].x;
-      im[n][k1 * RS + t] = v[n][k1].y;
+      L.st(n, k1 * RS + t, v[n][k1]);
     }
   lds_barrier();
   float2 b[NT][8];
@@ -205,7 +228,7 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const int a = (w * SUB + u) * RS + l + 64 * j;
-        b[n][u * NW + j] = make_float2(re[n][a], im[n][a]);
+        b[n][u * NW + j] = L.ld(n, a);
       }
   // stage 2: radix-NW over j, twiddle W_S^{m1 l}
   {
@@ -229,22 +252,15 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   wave_lds_order();
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float* wr = re[n] + w * 576;
-    float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int g = 0; g < 8; ++g) {
-      wr[g * 72 + l] = b[n][g].x;
-      wi[g * 72 + l] = b[n][g].y;
-    }
+    for (int g = 0; g < 8; ++g) L.st(n, w * 576 + g * 72 + l, b[n][g]);
   }
   wave_lds_order();
   const int grp = l >> 3, p = l & 7;
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const float* wr = re[n] + w * 576;
-    const float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) b[n][q] = make_float2(wr[grp * 72 + p + 8 * q], wi[grp * 72 + p + 8 * q]);
+    for (int q = 0; q < 8; ++q) b[n][q] = L.ld(n, w * 576 + grp * 72 + p + 8 * q);
   }
   // stage 3: radix-8 over q, twiddle W_64^{p r}
 #pragma unroll
@@ -258,35 +274,28 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   wave_lds_order();
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float* wr = re[n] + w * 576;
-    float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      wr[grp * 72 + r * 9 + p] = b[n][r].x;
-      wi[grp * 72 + r * 9 + p] = b[n][r].y;
-    }
+    for (int r = 0; r < 8; ++r) L.st(n, w * 576 + grp * 72 + r * 9 + p, b[n][r]);
   }
   wave_lds_order();
   const int rr = l & 7;
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const float* wr = re[n] + w * 576;
-    const float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int pp = 0; pp < 8; ++pp)
-      v[n][pp] = make_float2(wr[grp * 72 + rr * 9 + pp], wi[grp * 72 + rr * 9 + pp]);
+    for (int pp = 0; pp < 8; ++pp) v[n][pp] = L.ld(n, w * 576 + grp * 72 + rr * 9 + pp);
   }
   // stage 4: radix-8 over p -> slot s
 #pragma unroll
   for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
 }
 
-template <int N, bool INV, int NT, class TWT>
+template <int N, bool INV, int NT, bool IL = false, class TWT>
 __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
                                               int t, TWT tw) {
   using F = FS<N>;
   constexpr int NW = F::NW, SUB = F::SUB, RS = F::RS;
   const int w = t >> 6, l = t & 63;
+  const FsLds<IL, NT> L{re, im};
   const auto twp = opaque_offset(tw);
   const int grp = l >> 3, rr = l & 7;
   // stage 4^T: radix-8 over s -> p
@@ -296,23 +305,16 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
   wave_lds_order();
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float* wr = re[n] + w * 576;
-    float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int pp = 0; pp < 8; ++pp) {
-      wr[grp * 72 + rr * 9 + pp] = v[n][pp].x;
-      wi[grp * 72 + rr * 9 + pp] = v[n][pp].y;
-    }
+    for (int pp = 0; pp < 8; ++pp) L.st(n, w * 576 + grp * 72 + rr * 9 + pp, v[n][pp]);
   }
   wave_lds_order();
   const int p = l & 7;
   float2 b[NT][8];
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const float* wr = re[n] + w * 576;
-    const float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int r = 0; r < 8; ++r) b[n][r] = make_float2(wr[grp * 72 + r * 9 + p], wi[grp * 72 + r * 9 + p]);
+    for (int r = 0; r < 8; ++r) b[n][r] = L.ld(n, w * 576 + grp * 72 + r * 9 + p);
   }
   // stage 3^T: twiddle W_64^{p r}, radix-8 over r -> q
   {
@@ -326,21 +328,14 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
   wave_lds_order();
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    float* wr = re[n] + w * 576;
-    float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      wr[grp * 72 + p + 8 * q] = b[n][q].x;
-      wi[grp * 72 + p + 8 * q] = b[n][q].y;
-    }
+    for (int q = 0; q < 8; ++q) L.st(n, w * 576 + grp * 72 + p + 8 * q, b[n][q]);
   }
   wave_lds_order();
 #pragma unroll
   for (int n = 0; n < NT; ++n) {
-    const float* wr = re[n] + w * 576;
-    const float* wi = im[n] + w * 576;
 #pragma unroll
-    for (int g = 0; g < 8; ++g) b[n][g] = make_float2(wr[g * 72 + l], wi[g * 72 + l]);
+    for (int g = 0; g < 8; ++g) b[n][g] = L.ld(n, w * 576 + g * 72 + l);
   }
   // stage 2^T: twiddle W_S^{m1 l}, radix-NW over m1 -> j
   {
@@ -369,14 +364,13 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
 #pragma unroll
       for (int j = 0; j < NW; ++j) {
         const int a = (w * SUB + u) * RS + l + 64 * j;
-        re[n][a] = b[n][u * NW + j].x;
-        im[n][a] = b[n][u * NW + j].y;
+        L.st(n, a, b[n][u * NW + j]);
       }
   lds_barrier();
 #pragma unroll
   for (int n = 0; n < NT; ++n)
 #pragma unroll
-    for (int k1 = 0; k1 < 8; ++k1) v[n][k1] = make_float2(re[n][k1 * RS + t], im[n][k1 * RS + t]);
+    for (int k1 = 0; k1 < 8; ++k1) v[n][k1] = L.ld(n, k1 * RS + t);
   lds_barrier();
   // stage 1^T: twiddle W_N^{k1 t}, radix-8 over k1 -> i
   {
@@ -389,22 +383,22 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
 }
 
 // single-transform forms
-template <int N, bool INV, int NZJ, class TWT>
+template <int N, bool INV, int NZJ, bool IL = false, class TWT>
 __device__ __forceinline__ void fs_nat2perm(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
                                             TWT tw) {
   float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
   float* const rr[1] = {re};
   float* const ii[1] = {im};
-  fs_nat2perm_n<N, INV, NZJ, 1>(vv, rr, ii, t, tw);
+  fs_nat2perm_n<N, INV, NZJ, 1, IL>(vv, rr, ii, t, tw);
 }
 
-template <int N, bool INV, class TWT>
+template <int N, bool INV, bool IL = false, class TWT>
 __device__ __forceinline__ void fs_perm2nat(float2 (&v)[8], float* __restrict__ re, float* __restrict__ im, int t,
                                             TWT tw) {
   float2(&vv)[1][8] = *reinterpret_cast<float2(*)[1][8]>(&v);
   float* const rr[1] = {re};
   float* const ii[1] = {im};
-  fs_perm2nat_n<N, INV, 1>(vv, rr, ii, t, tw);
+  fs_perm2nat_n<N, INV, 1, IL>(vv, rr, ii, t, tw);
 }
 
 }  // namespace tdtm

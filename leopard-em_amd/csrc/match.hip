@@ -326,7 +326,7 @@ __global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict
     v[i] = z;
   }
   if constexpr (FOUR) {
-    fs_nat2perm<MF, false, NZJ>(v, lre, lim, t, tw);
+    fs_nat2perm<MF, false, NZJ, true>(v, lre, lim, t, tw);
     __syncthreads();  // other waves may still use their rows for the wave-local exchanges
   } else {
     fft_wg<M, V, false, NZJ>(v, lre, lim, t, tw);
@@ -394,7 +394,7 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
   for (int n = 0; n < NT; ++n)
 #pragma unroll
     for (int i = 0; i < V; ++i) v[n][i] = i < NL ? in[n][i] : make_float2(0.f, 0.f);
-  fs_nat2perm_n<HH, false, NZJ, NT>(v, re, im, t, tw);
+  fs_nat2perm_n<HH, false, NZJ, NT, true>(v, re, im, t, tw);
   if constexpr (NX > 0) {
     float2(&nx)[NX][NL] = *reinterpret_cast<float2(*)[NX][NL]>(&in);
     colpass_load<HH, NZJ, NX>(nx, Rb, hs, G, g, c, t, k + NT, klast);
@@ -405,7 +405,7 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
     for (int i = 0; i < V; ++i) v[n][i] = c_mulc(xi[i], v[n][i]);
     if (dc) v[n][0] = make_float2(0.f, 0.f);
   }
-  fs_perm2nat_n<HH, true, NT>(v, re, im, t, tw);
+  fs_perm2nat_n<HH, true, NT, true>(v, re, im, t, tw);
   // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
   // tile rows are Gs tiles apart
   constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
@@ -830,7 +830,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
     float* lre = cur + r * C::RSTR;
     float* lim = lre + LF;
     if constexpr (C::FOUR) {
-      fs_perm2nat<MF, true>(v, lre, lim, t, tw);
+      fs_perm2nat<MF, true, true>(v, lre, lim, t, tw);
     } else {
       fft_wg<M, V, true>(v, lre, lim, t, tw);
       lds_barrier();
