@@ -382,11 +382,15 @@ __device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // from k + NT, clamped to klast) are loaded into `in` right after the forward
 // transforms, so their latency overlaps the inverse transforms and this iteration's
 // stores.
-template <int HH, int NZJ, int NT, int NX, bool FULL, class TWT>
+// TS (two columns per block, one correlation per iteration): the result goes through LDS
+// so that every thread stores whole rows (both columns, 16 bytes) and 4 consecutive lanes
+// fill one 64-byte tile -- full-tile stores instead of 8-byte stores into tile halves.
+// xs = the block's LDS (the two columns' transform areas, >= 2 * HH float2).
+template <int HH, int NZJ, int NT, int NX, bool FULL, bool TS, class TWT>
 __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
                                              int hs, int G, int Gs, int g, int c, int t, int k, int klast, int ystore,
                                              bool dc, const float2 (&xi)[V], float* const (&re)[NT],
-                                             float* const (&im)[NT], TWT tw, float2* __restrict__ Gb) {
+                                             float* const (&im)[NT], TWT tw, float2* __restrict__ Gb, float2* xs) {
   constexpr int T = HH / V;
   constexpr int NL = NZJ < V ? NZJ : V;
   float2 v[NT][V];
@@ -406,31 +410,43 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
     if (dc) v[n][0] = make_float2(0.f, 0.f);
   }
   fs_perm2nat_n<HH, true, NT, true>(v, re, im, t, tw);
-  // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
-  // tile rows are Gs tiles apart
-  constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
+  if constexpr (TS) {
+    static_assert(NT == 1, "transposed stores take one correlation per iteration");
+    // column c's rows into plane c (lane-consecutive rows: conflict free), then each
+    // thread takes 4 rows, both columns
 #pragma unroll
-  for (int n = 0; n < NT; ++n) {
-    float2* gout = Gb + (long)(k + n) * HH * Gs * CW + ((long)(t / TR) * Gs + g) * TR * CW + (t % TR) * CW + c;
+    for (int i = 0; i < V; ++i) xs[c * HH + t + i * T] = v[0][i];
+    lds_barrier();
+    const int q = c * T + t;  // thread index in the block (2 T threads)
+    float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * CW) + (long)g * TR;
 #pragma unroll
-    for (int i = 0; i < V; ++i)
-      if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * Gs] = v[n][i];
+    for (int j = 0; j < HH / (2 * T); ++j) {
+      const int y = q + j * 2 * T;
+      const float2 a = xs[y], b = xs[HH + y];
+      if (FULL || y < ystore) gout[((long)(y / TR) * Gs) * TR + (y % TR)] = make_float4(a.x, a.y, b.x, b.y);
+    }
+    lds_barrier();  // the next iteration's exchanges reuse the planes
+  } else {
+    // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
+    // tile rows are Gs tiles apart
+    constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
+#pragma unroll
+    for (int n = 0; n < NT; ++n) {
+      float2* gout = Gb + (long)(k + n) * HH * Gs * CW + ((long)(t / TR) * Gs + g) * TR * CW + (t % TR) * CW + c;
+#pragma unroll
+      for (int i = 0; i < V; ++i)
+        if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * Gs] = v[n][i];
+    }
   }
 }
 
-// Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
-// conjugate against the image spectrum column, inverse FFT along y; tiled store.
-// Sizes >= 512 use the four-step transform (the product is formed in its permuted
-// order, the image column is held in registers in that order) with NT correlations per
-// iteration; smaller sizes use the Stockham passes.  CWB = columns per block (1 or the
-// CW columns of a tile).
 // Column pass: pruned forward FFT along y (HS of HH rows non-zero), multiply by the
 // conjugate against the image spectrum column, inverse FFT along y; tiled store.
 // Sizes >= 512 use the four-step transform with NT correlations per iteration; smaller
 // sizes use the Stockham passes.  CWB = columns per block (1 or the CW columns of a
 // tile).  TWM = 1 reads the twiddles from an LDS copy of the table entries the
 // four-step transforms of size HH use; FULL = every row is stored (ystore == HH).
-template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL>
+template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL, bool TS = false>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int Gs, int K, int kchunk, int ystore,
                                                          const float2* __restrict__ tw_global, float2* __restrict__ Gb) {
@@ -487,6 +503,8 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   }
   const bool dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
   float* base = lds + cl * NT * 2 * LF;
+  float2* lds2 = reinterpret_cast<float2*>(lds);
+  static_assert(!TS || (CWB == CW && NT == 1 && FOUR && 2 * HH <= CW * LF), "transposed stores: layout");
   if constexpr (FOUR) {
     float* const re1[1] = {base};
     float* const im1[1] = {base + LF};
@@ -502,26 +520,26 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       // wait for the inputs then leaves the stores in flight)
       if (k + 1 < k1) {
         colpass_load<HF, NZJ, 2>(in, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+        colpass_four<HF, NZJ, 2, 2, FULL, TS>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
         k += 2;
       }
       for (; k + 1 < k1; k += 2)
-        colpass_four<HF, NZJ, 2, 2, FULL>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb);
+        colpass_four<HF, NZJ, 2, 2, FULL, TS>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
       // a single last correlation: its input was prefetched into in[0], unless the chunk
       // holds only this one
       if (k < k1) {
         if (k == k0) colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 0, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 0, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
       }
     } else {
       const int klast = k1 - 1;
       if (k < k1) {  // peeled first iteration (see the paired loop)
         colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
         ++k;
       }
       for (; k < k1; ++k)
-        colpass_four<HF, NZJ, 1, 1, FULL>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb);
+        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
     }
   } else {
     float* lre = base;
@@ -1221,6 +1239,7 @@ struct tdtm_match_plan {
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
   int opt_colpass_twiddles = 1; // four-step column transforms: 0 global table, 1 LDS copy
+  int opt_colpass_store = 0;    // two-column blocks: 0 direct 8-byte stores, 1 whole rows via LDS
   int opt_rowpass_staging = 3;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat,
                                 // 3: synchronous de-interleaved with batched loads
   int opt_rowpass_twiddles = 1; // 0: twiddles from the global table, 1: LDS copy (staging 0 / 3 only)
@@ -1326,7 +1345,15 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
   using Kern = void (*)(const float2*, const float2*, int, int, int, int, int, int, const float2*, float2*);
   Kern kern = nullptr;
-#define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_) kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_>
+#define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                       \
+  do {                                                                                             \
+    if constexpr (CWB_ == CW && NT_ == 1 && FOUR) {                                                \
+      if (p->opt_colpass_store == 1) kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_, true>;       \
+      else kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_, false>;                              \
+    } else {                                                                                       \
+      kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_, false>;                                    \
+    }                                                                                              \
+  } while (0)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
@@ -1545,6 +1572,9 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     p->opt_colpass_columns = value;
   } else if (k == "colpass_pair") {
     p->opt_colpass_pair = value != 0;
+  } else if (k == "colpass_store") {
+    if (value < 0 || value > 1) return fail("colpass_store must be 0 or 1");
+    p->opt_colpass_store = value;
   } else if (k == "colpass_twiddles") {
     if (value < 0 || value > 1) return fail("colpass_twiddles must be 0 or 1");
     p->opt_colpass_twiddles = value;

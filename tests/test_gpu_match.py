@@ -156,6 +156,7 @@ def test_match_peak_list_identical(cuda_device):
     {"rowpass_rows": 4, "rowpass_twiddles": 0, "rowpass_staging": 2}, {"rowpass_rows": 4}, {"colpass_chunks": 3}, {"colpass_columns": 1, "colpass_pair": 0},
     {"colpass_twiddles": 0}, {"colpass_twiddles": 0, "colpass_pair": 1}, {"colpass_columns": 1, "colpass_chunks": 5},
     {"rowpass_rows": 2}, {"rowpass_rows": 1}, {"rowpass_rows": 1, "rowpass_twiddles": 0, "rowpass_staging": 0},
+    {"colpass_store": 1}, {"colpass_store": 1, "colpass_twiddles": 0},
     {"phase_timing": 1},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
