@@ -290,28 +290,31 @@ __global__ void k_proj_norm(const float* __restrict__ part, int hs, int H, int W
 }
 
 // Zero-padded r2c along x (WS points, HS non-zero) of the normalised projection rows.
+// A block transforms RR = 4 consecutive rows at once (one row per T threads), so every
+// tile column gets 64 contiguous bytes from one block; the two blocks of each 8-row
+// group (a 128-byte line of Rb per tile column) get ids 8 apart, so they run on the same
+// XCD (workgroups go to the 8 XCDs round robin) and complete the lines in its L2.
 template <int WS, int NZJ>
-__global__ __launch_bounds__((WS / 2) / V) void k_rowfwd(const float* __restrict__ P, const float2* __restrict__ ae,
-                                                        int hs, int G, const float2* __restrict__ tw,
-                                                        float2* __restrict__ Rb) {
+__global__ __launch_bounds__(4 * (WS / 2) / V) void k_rowfwd(const float* __restrict__ P, const float2* __restrict__ ae,
+                                                            int hs, int G, const float2* __restrict__ tw,
+                                                            float2* __restrict__ Rb) {
+  constexpr int RR = 4;
   constexpr int M = WS / 2;
   constexpr int T = M / V;
   constexpr bool FOUR = UseFour<M>::value;
   constexpr int MF = FOUR ? M : 512;
   constexpr int LF = FOUR ? FS<MF>::floats : LdsSize<M>::floats;
-  __shared__ float lds[2 * LF];
-  float* lre = lds;
-  float* lim = lds + LF;
-  // rows 8m .. 8m+7 share the 128-byte lines of Rb (16 bytes each per tile column): with
-  // hs a multiple of 64 their blocks get ids 8 apart, so they run on the same XCD
-  // (workgroups go to the 8 XCDs round robin) and complete the lines in its L2
-  int y = blockIdx.x;
+  __shared__ float lds[RR * 2 * LF];
+  const int r = threadIdx.x / T, t = threadIdx.x % T;
+  float* lre = lds + r * 2 * LF;
+  float* lim = lre + LF;
+  int y0 = blockIdx.x * RR;
   if (hs % 64 == 0) {
-    const int b = blockIdx.x;
-    y = ((((b >> 6) << 3) + (b & 7)) << 3) + ((b >> 3) & 7);
+    const int b = blockIdx.x, q = b >> 3;
+    y0 = 8 * ((q >> 1) * 8 + (b & 7)) + RR * (q & 1);
   }
+  const int y = y0 + r;
   const int k = blockIdx.y;
-  const int t = threadIdx.x;
   const float2 s = ae[k];
   const float2* prow = reinterpret_cast<const float2*>(P + ((long)k * hs + y) * hs);
   float2 v[V];
@@ -1317,8 +1320,8 @@ static int dispatch_proj(tdtm_match_plan* p, hipStream_t st) {
 
 template <int WS, int NZJ>
 static void launch_rowfwd_n(tdtm_match_plan* p, int K, hipStream_t st) {
-  dim3 grid(p->h, K);
-  k_rowfwd<WS, NZJ><<<grid, (WS / 2) / V, 0, st>>>(p->P, p->ae, p->h, p->G, p->tw, p->Rb);
+  dim3 grid(p->h / 4, K);  // 4 rows per block (h is a power of two >= 32)
+  k_rowfwd<WS, NZJ><<<grid, 4 * (WS / 2) / V, 0, st>>>(p->P, p->ae, p->h, p->G, p->tw, p->Rb);
 }
 
 template <int WS>
