@@ -45,10 +45,49 @@ def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
     tdft = pp.volume_to_rfft_fourier_slice(vol).to(torch.complex64).contiguous()
     mg = synthetic.make_micrograph(t, seed, (H, W), n_particles, 12200.0, 11800.0, 30.0, snr=0.05,
                                    defocus_spread=3000.0, device=device)
-    img_dft, wf = pp.prepare_image(mg.image.to(device), h)
+    image = mg.image.to(device=device, dtype=torch.float32).contiguous()
+    img_dft, wf = pp.prepare_image(image, h)
     dfs = torch.arange(-1200.0, 1201.0, 200.0)
     ctf = pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]), device=device)
-    return img_dft, tdft, ctf, wf.to(device), dfs, vol
+    return img_dft, tdft, ctf, wf.to(device), dfs, vol, image
+
+
+def time_preparation(image, vol, h, dfs, device, reps=5):
+    """Device time of the once-per-search input preparation (image spectrum + whitening,
+    template 3D rfft, CTF stack) and its rate against the minimum HBM traffic: the
+    input read once and the output written once."""
+    import torch
+
+    from leopard_em_amd.utils import preprocessing as pp
+
+    stream = torch.cuda.current_stream(device)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize(device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    H, W = image.shape
+    n = vol.shape[0]
+    out = {}
+    jobs = {
+        "prepare_image": (lambda: pp.prepare_image_device(image, h), 4 * H * W + 8 * H * (W // 2 + 1)),
+        "volume_rfft": (lambda: pp.volume_rfft_device(vol), 4 * n ** 3 + 8 * n * n * (n // 2 + 1)),
+        "ctf_stack": (lambda: pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]),
+                                                  device=device), 4 * dfs.numel() * h * (h // 2 + 1)),
+    }
+    for name, (fn, nbytes) in jobs.items():
+        ms = timed(fn)
+        gbs = nbytes / (ms / 1e3) / 1e9
+        out[name] = {"ms": ms, "min_bytes": nbytes, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    return out
 
 
 def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
@@ -105,6 +144,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (reference kernel for HBM byte counters)")
     ap.add_argument("--no-phase-timing", action="store_true")
+    ap.add_argument("--no-prep-timing", action="store_true",
+                    help="skip timing the input preparation kernels (synthetic inputs only)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -123,7 +164,7 @@ def main():
     print(f"[bench] building {args.inputs} inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
     if args.inputs == "synthetic":
         n_blobs = 5000 if h == 512 else max(100, int(5000 * (h / 512) ** 3))
-        img_dft, tdft, ctf, wf, dfs, _ = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
+        img_dft, tdft, ctf, wf, dfs, vol, image = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
     else:
         gen = torch.Generator(device=dev).manual_seed(0)
         img_dft = torch.randn((H, W // 2 + 1), dtype=torch.complex64, device=dev, generator=gen)
@@ -192,6 +233,12 @@ def main():
 
     traffic = read_traffic(TRAFFIC_SUMMARY) if (H, W, h, B, D) == (4096, 4096, 512, 8, 13) else None
 
+    prep = None
+    if args.inputs == "synthetic" and not args.no_prep_timing:
+        prep = time_preparation(image, vol, h, dfs, dev)
+        print("[bench] preparation ms: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in prep.items()),
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         print("[bench] CPU baseline (float32 oracle) ...", file=sys.stderr, flush=True)
@@ -236,6 +283,7 @@ def main():
             "phases_ms_per_step": phases,
         },
         "cpu_baseline": cpu,
+        "preparation": prep,
     }
     print(json.dumps(line), flush=True)
 

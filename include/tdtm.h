@@ -160,6 +160,60 @@ int tdtm_refine_particle(tdtm_match_plan_t plan, const void* template_dft, const
                          const float* projective_filter, const float* corr_mean, const float* corr_std,
                          float* out, void* stream);
 
+/* ---------------------------------------------------------------- input preparation
+ * Once-per-search inputs built on the device.  The reference builds them on the host
+ * in MatchTemplateManager.make_backend_core_function_kwargs
+ * (src/leopard_em/pydantic_models/managers/match_template_manager.py:140-218).
+ */
+
+/*
+ * Unnormalised forward 2D real FFT: image float32 (H, W) -> out complex64 (H, W/2+1),
+ * as torch.fft.rfftn(image) (match_template_manager.py:160).  H, W powers of two in
+ * [32, 4096].
+ */
+int tdtm_rfft2(const float* image, int H, int W, void* out, void* stream);
+
+/*
+ * fftshift -> rfftn -> fftshift(dims 0, 1) of a cubic volume: volume float32 (n, n, n)
+ * -> out complex64 (n, n, n/2+1).  Replaces volume_to_rfft_fourier_slice
+ * (src/leopard_em/utils/image_processing.py:179-202).  n a power of two in [32, 512].
+ */
+int tdtm_volume_rfft(const float* volume, int n, void* out, void* stream);
+
+/* Device workspace bytes tdtm_prepare_image needs (-1: unsupported size or bin count). */
+int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins);
+
+/*
+ * Pre-processed image spectrum and template-grid whitening filter with the whitening
+ * filter on or off and the bandpass / arbitrary-curve filters off:
+ *   X = rfft2(image), X[0,0] = 0                      (match_template_manager.py:160-161)
+ *   whitening: radial power of X in num_freq_bins bins on [0, max_freq] cycles / pixel
+ *   (0 = min(H, W)/2 + 1 bins) -> curve 1/sqrt(P(f)), max-normalised, linearly
+ *   interpolated onto an rfft grid, 1 beyond max_freq  (config/correlation_filters.py:49-108);
+ *   all ones when whitening = 0
+ *   image_dft = X * whitening(H, W) normalised to unit real-space variance
+ *                                                     (utils/image_processing.py:16-75)
+ *   whitening_template = whitening on the (h, h/2+1) grid  (match_template_manager.py:169-177)
+ * image float32 (H, W); image_dft complex64 (H, W/2+1); whitening_template float32
+ * (h, h/2+1); workspace of tdtm_prepare_image_workspace_bytes(H, W, num_freq_bins)
+ * device bytes.
+ */
+int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
+                       void* image_dft, float* whitening_template, void* workspace, void* stream);
+
+/*
+ * CTF x B-envelope filter stack, out float32 (num_pixel, num_defocus, h, h/2+1):
+ * replaces calculate_ctf_filter_stack_full_args (src/leopard_em/utils/ctf_utils.py:19-116)
+ * with torch_ctf's sign convention (CTF(0) = +amplitude contrast).  defocus = offsets +
+ * (u + v)/2, astigmatism |u - v|/2 at defocus_angle degrees, Cs scaled per pixel size
+ * as Cs / ((p + dp)/p)^4 (utils/search_utils.py:27-28).  defocus_offsets (num_defocus)
+ * and pixel_offsets (num_pixel) float32 on the device; ctf_params host doubles as for
+ * tdtm_refine_particle.
+ */
+int tdtm_ctf_filter_stack(int h, int num_defocus, int num_pixel, double defocus_u, double defocus_v,
+                          double defocus_angle, const float* defocus_offsets, const float* pixel_offsets,
+                          const double* ctf_params, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,6 +24,7 @@
 #include <string>
 #include <vector>
 
+#include "common.hpp"
 #include "fft.hpp"
 #include "fft4.hpp"
 #include "tdtm.h"
@@ -35,26 +36,6 @@ constexpr int CW = 2;  // columns per column-pass block (tile width)
 constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
 constexpr int TDTM_LDS_BYTES = 160 * 1024;  // LDS one workgroup may allocate on gfx950
 
-thread_local std::string g_err;
-
-static int fail(const std::string& msg) {
-  g_err = msg;
-  return 1;
-}
-
-#define TDTM_HIP(call)                                                                  \
-  do {                                                                                  \
-    hipError_t e_ = (call);                                                             \
-    if (e_ != hipSuccess) return fail(std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define TDTM_LAUNCHED()                                                                 \
-  do {                                                                                  \
-    hipError_t e_ = hipGetLastError();                                                  \
-    if (e_ != hipSuccess) return fail(std::string("kernel launch: ") + hipGetErrorString(e_)); \
-  } while (0)
-
-static inline bool is_pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
 // ------------------------------------------------------------------------- kernels
 __global__ void k_tile_image(const float2* __restrict__ X, int H, int Wr, int G, float scale,
@@ -1230,6 +1211,34 @@ __global__ void k_refine_select(const RefineBest* __restrict__ slots, int nbatch
 
 }  // namespace tdtm
 
+// CTF x B-envelope stack (C, D, h, h/2+1) from the optics parameters ctf_params =
+// {voltage kV, Cs mm, amplitude contrast, phase shift deg, pixel size A, B-factor A^2}
+// (utils/ctf_utils.py:19-116): defocus = offsets + (u + v)/2, astigmatism |u - v|/2
+// at `defocus_angle` degrees, Cs / ((p + dp)/p)^4 per pixel size, relativistic
+// wavelength from CODATA constants.
+namespace tdtm {
+static int launch_ctf_stack(int h, int D, int C, double defocus_u, double defocus_v, double defocus_angle,
+                            const float* defocus_offsets, const float* pixel_offsets, const double* ctf_params,
+                            float* out, hipStream_t st) {
+  const double voltage = ctf_params[0], cs_mm = ctf_params[1], ac = ctf_params[2], phase = ctf_params[3];
+  const double px = ctf_params[4], bfac = ctf_params[5];
+  if (!(voltage > 0) || !(px > 0) || !(ac >= 0 && ac < 1)) return fail("invalid CTF parameters");
+  const double v = voltage * 1e3;
+  const double lam = 6.62607015e-34 /
+                     std::sqrt(2 * 9.1093837015e-31 * 1.602176634e-19 * v *
+                               (1 + 1.602176634e-19 * v / (2 * 9.1093837015e-31 * 299792458.0 * 299792458.0))) *
+                     1e10;
+  const long tot = (long)h * (h / 2 + 1) * D * C;
+  k_ctf_stack<<<(tot + 255) / 256, 256, 0, st>>>(h, D, C, 0.5 * (defocus_u + defocus_v),
+                                                 0.5 * std::fabs(defocus_u - defocus_v), defocus_angle * M_PI / 180.0,
+                                                 defocus_offsets, pixel_offsets, lam, cs_mm * 1e7, px,
+                                                 std::atan(ac / std::sqrt(1 - ac * ac)), phase * M_PI / 180.0, bfac,
+                                                 out);
+  TDTM_LAUNCHED();
+  return 0;
+}
+}  // namespace tdtm
+
 // --------------------------------------------------------------------------- plan
 using namespace tdtm;
 
@@ -1478,7 +1487,16 @@ extern "C" {
 
 int tdtm_abi_version(void) { return 1; }
 
-const char* tdtm_last_error(void) { return g_err.c_str(); }
+int tdtm_ctf_filter_stack(int h, int num_defocus, int num_pixel, double defocus_u, double defocus_v,
+                          double defocus_angle, const float* defocus_offsets, const float* pixel_offsets,
+                          const double* ctf_params, float* out, void* stream) {
+  if (!defocus_offsets || !pixel_offsets || !ctf_params || !out) return fail("null argument");
+  if (h < 2 || (h & 1) || num_defocus < 1 || num_pixel < 1) return fail("bad sizes");
+  return launch_ctf_stack(h, num_defocus, num_pixel, defocus_u, defocus_v, defocus_angle, defocus_offsets,
+                          pixel_offsets, ctf_params, out, static_cast<hipStream_t>(stream));
+}
+
+const char* tdtm_last_error(void) { return last_error().c_str(); }
 
 int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_plan_t* out) {
   if (!out) return fail("out_plan is NULL");
@@ -1526,28 +1544,7 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
     delete p;
     return 1;
   }
-  // tw[m] = e^{-2 pi i m / TW_N}: evaluated on the first octant and mapped by symmetry,
-  // so quarter and eighth turns are exact and the table has the circle's symmetries
-  std::vector<float2> tw(TW_N);
-  constexpr int Q = TW_N / 4;
-  for (int m = 0; m < TW_N; ++m) {
-    const int q = m / Q, r = m % Q;
-    double c, sn;  // cos, sin of 2 pi r / TW_N, r in [0, Q)
-    if (2 * r <= Q) {
-      const double a = 2.0 * M_PI * (double)r / (double)TW_N;
-      c = std::cos(a);
-      sn = std::sin(a);
-    } else {
-      const double a = 2.0 * M_PI * (double)(Q - r) / (double)TW_N;
-      c = std::sin(a);
-      sn = std::cos(a);
-    }
-    double cq = c, sq = sn;  // rotate by q quarter turns
-    if (q == 1) cq = -sn, sq = c;
-    if (q == 2) cq = -c, sq = -sn;
-    if (q == 3) cq = sn, sq = -c;
-    tw[m] = make_float2((float)cq, (float)-sq);
-  }
+  const std::vector<float2> tw = twiddle_table();
   hipError_t e = hipMemcpy(p->tw, tw.data(), TW_N * sizeof(float2), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     plan_free(p);
@@ -1755,21 +1752,11 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
   // image tiles (match plan semantics: scale 1/(H W))
   if (tdtm_match_set_image(p, particle_dft, stream)) return 1;
   // per-particle CTF stack and projective filter
-  const double voltage = ctf_params[0], cs_mm = ctf_params[1], ac = ctf_params[2], phase = ctf_params[3];
-  const double px = ctf_params[4], bfac = ctf_params[5];
-  const double v = voltage * 1e3;
-  const double lam = 6.62607015e-34 /
-                     std::sqrt(2 * 9.1093837015e-31 * 1.602176634e-19 * v *
-                               (1 + 1.602176634e-19 * v / (2 * 9.1093837015e-31 * 299792458.0 * 299792458.0))) *
-                     1e10;
+  if (launch_ctf_stack(p->h, p->D, p->C, defocus_u, defocus_v, defocus_angle, defocus_offsets, pixel_offsets,
+                       ctf_params, p->ctfbuf, st))
+    return 1;
   const long plane = (long)p->h * p->wr;
   const long tot = plane * p->D * p->C;
-  k_ctf_stack<<<(tot + 255) / 256, 256, 0, st>>>(p->h, p->D, p->C, 0.5 * (defocus_u + defocus_v),
-                                                 0.5 * std::fabs(defocus_u - defocus_v), defocus_angle * M_PI / 180.0,
-                                                 defocus_offsets, pixel_offsets, lam, cs_mm * 1e7, px,
-                                                 std::atan(ac / std::sqrt(1 - ac * ac)), phase * M_PI / 180.0, bfac,
-                                                 p->ctfbuf);
-  TDTM_LAUNCHED();
   k_filters<<<(tot + 255) / 256, 256, 0, st>>>(p->ctfbuf, projective_filter, plane, tot, p->filt);
   TDTM_LAUNCHED();
   const int ystore = ((crop_h + TR - 1) / TR) * TR;

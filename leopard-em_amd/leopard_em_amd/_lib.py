@@ -37,6 +37,13 @@ SIGNATURES = {
     "tdtm_refine_particle": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tdtm_rfft2": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "tdtm_volume_rfft": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "tdtm_prepare_image_workspace_bytes": (c_int64, [c_int, c_int, c_int]),
+    "tdtm_prepare_image": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.c_double, c_void_p, c_void_p,
+                                   c_void_p, c_void_p]),
+    "tdtm_ctf_filter_stack": (c_int, [c_int, c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 

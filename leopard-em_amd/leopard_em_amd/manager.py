@@ -37,7 +37,8 @@ from .analysis import (
 )
 from .backend.core_match_template import core_match_template, core_match_template_resumable
 from .io.mrc import load_mrc_image, load_mrc_volume, read_mrc_header, write_mrc_from_tensor
-from .utils.preprocessing import ctf_filter_stack, preprocess_image, volume_to_rfft_fourier_slice, whitening_filter
+from .utils.preprocessing import (ctf_filter_stack, prepare_image_device, preprocess_image,
+                                  volume_to_rfft_fourier_slice, whitening_filter)
 from .utils.search import get_search_tensors, get_uniform_euler_angles
 
 
@@ -287,23 +288,31 @@ class MatchTemplateManager:
             self.template_volume = load_mrc_volume(self.template_volume_path)
         image = torch.as_tensor(self.micrograph).to(torch.float32)
         template = torch.as_tensor(self.template_volume).to(torch.float32)
-        image_dft = torch.fft.rfftn(image)  # pylint: disable=not-callable
-        image_dft[0, 0] = 0
-        bandpass = torch.ones(image_dft.shape, dtype=torch.float32)
-        filt_img = self.preprocessing_filters.filter_for(image_dft, image_dft.shape)
-        filt_tmp = self.preprocessing_filters.filter_for(image_dft, (template.shape[-2], template.shape[-1] // 2 + 1))
-        image_pre = preprocess_image(image_dft, filt_img, bandpass, tuple(image.shape), tuple(image.shape))
         defocus_values = self.defocus_search_config.defocus_values
         pixel_size_offsets = torch.tensor([0.0], dtype=torch.float32)
         og = self.optics_group
+        pf = self.preprocessing_filters
+        # with a GPU the inputs are built by the library's kernels on the first search
+        # device; otherwise on the host with torch, as the reference does
+        dev = self.computational_config.gpu_devices[0] if torch.cuda.is_available() else torch.device("cpu")
+        if dev.type == "cuda":
+            image_pre, filt_tmp = prepare_image_device(image.to(dev), template.shape[-1], pf.whitening_enabled,
+                                                       pf.num_freq_bins, pf.max_freq)
+        else:
+            image_dft = torch.fft.rfftn(image)  # pylint: disable=not-callable
+            image_dft[0, 0] = 0
+            bandpass = torch.ones(image_dft.shape, dtype=torch.float32)
+            filt_img = pf.filter_for(image_dft, image_dft.shape)
+            filt_tmp = pf.filter_for(image_dft, (template.shape[-2], template.shape[-1] // 2 + 1))
+            image_pre = preprocess_image(image_dft, filt_img, bandpass, tuple(image.shape), tuple(image.shape))
         ctf = ctf_filter_stack(
             (template.shape[0], template.shape[0]), og.defocus_u, og.defocus_v, og.astigmatism_angle,
             defocus_values, pixel_size_offsets, voltage=og.voltage, spherical_aberration=og.spherical_aberration,
             amplitude_contrast_ratio=og.amplitude_contrast_ratio, phase_shift=og.phase_shift,
-            pixel_size=og.pixel_size, ctf_B_factor=og.ctf_B_factor)
+            pixel_size=og.pixel_size, ctf_B_factor=og.ctf_B_factor, device=dev)
         return {
             "image_dft": image_pre.to(torch.complex64),
-            "template_dft": volume_to_rfft_fourier_slice(template).to(torch.complex64),
+            "template_dft": volume_to_rfft_fourier_slice(template.to(dev)).to(torch.complex64),
             "ctf_filters": ctf,
             "whitening_filter_template": filt_tmp.to(torch.float32),
             "euler_angles": self.orientation_search_config.euler_angles.to(torch.float32),

@@ -4,7 +4,12 @@ Counterparts of what ``MatchTemplateManager.make_backend_core_function_kwargs``
 (``pydantic_models/managers/match_template_manager.py:140-218``) builds before it calls
 ``core_match_template``: the pre-processed image rfft, the template-grid whitening
 filter, the CTF filter stack and the fftshifted 3D template rfft.  These run once per
-micrograph on torch tensors (CPU or GPU) and are not part of the timed search.
+micrograph and are not part of the timed search.
+
+On CUDA (HIP) tensors the transforms and filters run as the library's own kernels
+(``tdtm_rfft2``, ``tdtm_volume_rfft``, ``tdtm_prepare_image``, ``tdtm_ctf_filter_stack``);
+there is no fallback, the call raises if the library is not built.  CPU tensors keep the
+host computation the reference does (torch on the CPU).
 
 * ``preprocess_image``            ``utils/image_processing.py:16-75``
 * ``volume_to_rfft_fourier_slice`` ``utils/image_processing.py:179-202``
@@ -15,9 +20,12 @@ micrograph on torch tensors (CPU or GPU) and are not part of the timed search.
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import torch
+
+from .. import _lib
 
 _H = 6.62607015e-34
 _ME = 9.1093837015e-31
@@ -46,6 +54,11 @@ def ctf_filter_stack(template_shape, defocus_u, defocus_v, astigmatism_angle, de
     defocus = offsets + (u+v)/2, astigmatism |u-v|/2 at ``astigmatism_angle`` degrees,
     Cs scaled per pixel size as Cs / (p/p0)^4 (``utils/search_utils.py:27-28``).
     """
+    if torch.device(device).type == "cuda":
+        return ctf_filter_stack_device(template_shape, defocus_u, defocus_v, astigmatism_angle, defocus_offsets,
+                                       pixel_size_offsets, voltage=voltage, spherical_aberration=spherical_aberration,
+                                       amplitude_contrast_ratio=amplitude_contrast_ratio, phase_shift=phase_shift,
+                                       pixel_size=pixel_size, ctf_B_factor=ctf_B_factor, device=device)
     dt = torch.float64
     dfs = torch.as_tensor(defocus_offsets, dtype=dt, device=device) + 0.5 * (defocus_u + defocus_v)
     astig = abs(defocus_u - defocus_v) / 2
@@ -111,6 +124,8 @@ def preprocess_image(image_rfft, cumulative_fourier_filters, bandpass_filter, fu
 
 def volume_to_rfft_fourier_slice(volume: torch.Tensor) -> torch.Tensor:
     """fftshift -> rfftn -> fftshift(dims 0, 1) of a cubic volume."""
+    if volume.is_cuda:
+        return volume_rfft_device(volume)
     v = torch.fft.fftshift(volume, dim=(0, 1, 2))
     v = torch.fft.rfftn(v, dim=(0, 1, 2))
     return torch.fft.fftshift(v, dim=(0, 1))
@@ -122,6 +137,8 @@ def prepare_image(image: torch.Tensor, template_size: int):
     Default filter configuration of the reference (whitening on, bandpass and
     arbitrary-curve filters off).
     """
+    if image.is_cuda:
+        return prepare_image_device(image, template_size)
     img_dft = torch.fft.rfftn(image.to(torch.float32))
     img_dft[0, 0] = 0
     wf_img = whitening_filter(img_dft, tuple(img_dft.shape))
@@ -129,3 +146,77 @@ def prepare_image(image: torch.Tensor, template_size: int):
     bp = torch.ones(img_dft.shape, dtype=torch.float32, device=img_dft.device)
     pre = preprocess_image(img_dft, wf_img, bp, tuple(image.shape), tuple(image.shape))
     return pre.to(torch.complex64), wf_tmp
+
+
+# ---------------------------------------------------------------- device kernels
+def _stream(device: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def rfft2_device(image: torch.Tensor) -> torch.Tensor:
+    """Unnormalised rfft2 of a (H, W) float32 CUDA image -> (H, W/2+1) complex64
+    (``tdtm_rfft2``; H, W powers of two in [32, 4096])."""
+    lib = _lib.load()
+    img = image.to(torch.float32).contiguous()
+    H, W = img.shape
+    out = torch.empty((H, W // 2 + 1), dtype=torch.complex64, device=img.device)
+    with torch.cuda.device(img.device):
+        _lib.check(lib.tdtm_rfft2(_lib.ptr(img), H, W, _lib.ptr(out), _stream(img.device)))
+    return out
+
+
+def volume_rfft_device(volume: torch.Tensor) -> torch.Tensor:
+    """fftshift -> rfftn -> fftshift(dims 0, 1) of a cubic float32 CUDA volume
+    (``tdtm_volume_rfft``; side a power of two in [32, 512])."""
+    lib = _lib.load()
+    vol = volume.to(torch.float32).contiguous()
+    n = vol.shape[0]
+    if vol.shape != (n, n, n):
+        raise ValueError(f"volume must be cubic, got {tuple(vol.shape)}")
+    out = torch.empty((n, n, n // 2 + 1), dtype=torch.complex64, device=vol.device)
+    with torch.cuda.device(vol.device):
+        _lib.check(lib.tdtm_volume_rfft(_lib.ptr(vol), n, _lib.ptr(out), _stream(vol.device)))
+    return out
+
+
+def prepare_image_device(image: torch.Tensor, template_size: int, whitening: bool = True,
+                         num_freq_bins: int | None = None, max_freq: float = 0.5):
+    """(pre-processed image rfft complex64, template-grid whitening float32) of a CUDA
+    image (``tdtm_prepare_image``): whitening on or off, bandpass and curve filters off."""
+    lib = _lib.load()
+    img = image.to(torch.float32).contiguous()
+    H, W = img.shape
+    nb = int(num_freq_bins or 0)
+    nbytes = lib.tdtm_prepare_image_workspace_bytes(H, W, nb)
+    if nbytes < 0:
+        raise _lib.TdtmError(f"unsupported image size {H}x{W} or bin count {nb}")
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=img.device)
+    out = torch.empty((H, W // 2 + 1), dtype=torch.complex64, device=img.device)
+    wf = torch.empty((template_size, template_size // 2 + 1), dtype=torch.float32, device=img.device)
+    with torch.cuda.device(img.device):
+        _lib.check(lib.tdtm_prepare_image(_lib.ptr(img), H, W, template_size, int(bool(whitening)), nb,
+                                          float(max_freq), _lib.ptr(out), _lib.ptr(wf), _lib.ptr(ws),
+                                          _stream(img.device)))
+    return out, wf
+
+
+def ctf_filter_stack_device(template_shape, defocus_u, defocus_v, astigmatism_angle, defocus_offsets,
+                            pixel_size_offsets, *, voltage=300.0, spherical_aberration=2.7,
+                            amplitude_contrast_ratio=0.07, phase_shift=0.0, pixel_size=1.0, ctf_B_factor=0.0,
+                            device="cuda") -> torch.Tensor:
+    """``ctf_filter_stack`` on the GPU (``tdtm_ctf_filter_stack``)."""
+    lib = _lib.load()
+    dev = torch.device(device)
+    h = int(template_shape[0])
+    if tuple(template_shape) != (h, h):
+        raise ValueError("the CTF stack is built for square templates")
+    dfo = torch.as_tensor(defocus_offsets, dtype=torch.float32).reshape(-1).to(dev).contiguous()
+    pxo = torch.as_tensor(pixel_size_offsets, dtype=torch.float32).reshape(-1).to(dev).contiguous()
+    params = (ctypes.c_double * 6)(voltage, spherical_aberration, amplitude_contrast_ratio, phase_shift,
+                                        pixel_size, ctf_B_factor)
+    out = torch.empty((pxo.numel(), dfo.numel(), h, h // 2 + 1), dtype=torch.float32, device=dev)
+    with torch.cuda.device(dev):
+        _lib.check(lib.tdtm_ctf_filter_stack(h, dfo.numel(), pxo.numel(), float(defocus_u), float(defocus_v),
+                                             float(astigmatism_angle), _lib.ptr(dfo), _lib.ptr(pxo), params,
+                                             _lib.ptr(out), _stream(dev)))
+    return out

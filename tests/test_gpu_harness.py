@@ -81,9 +81,12 @@ def test_manager_yaml_mrc_end_to_end(tmp_path, cuda_device):
     print(f"manager: {H}x{W}, h {h}, D 3, O {O}", flush=True)
     assert m.run_match_template(orientation_batch_size=8) is True
     r = m.match_template_result
-    ref = osr.core_match_template(kw["image_dft"], kw["template_dft"], kw["ctf_filters"],
-                                  kw["whitening_filter_template"], kw["defocus_values"], kw["pixel_values"],
-                                  kw["euler_angles"], orientation_batch_size=O)
+    # the manager builds its inputs on the GPU; the oracle search runs on CPU copies
+    assert kw["image_dft"].is_cuda and kw["template_dft"].is_cuda and kw["ctf_filters"].is_cuda
+    ck = {k: (v.cpu() if isinstance(v, torch.Tensor) else v) for k, v in kw.items()}
+    ref = osr.core_match_template(ck["image_dft"], ck["template_dft"], ck["ctf_filters"],
+                                  ck["whitening_filter_template"], ck["defocus_values"], ck["pixel_values"],
+                                  ck["euler_angles"], orientation_batch_size=O)
     crop = (slice(0, H - h + 1), slice(0, W - h + 1))
     scale = float(ref["mip"].abs().max())
     assert float((r.mip - ref["mip"][crop]).abs().max()) / scale < 1e-4

@@ -45,3 +45,28 @@ def test_plan_size_validation_errors_before_any_device_call():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(_lib.TdtmError, match="not built"):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libtdtm.so not built")
+def test_prep_validation_errors_before_any_device_call():
+    lib = _lib.load()
+    assert lib.tdtm_prepare_image_workspace_bytes(100, 128, 0) == -1
+    assert lib.tdtm_prepare_image_workspace_bytes(4096, 8192, 0) == -1
+    assert lib.tdtm_prepare_image_workspace_bytes(64, 64, 1) == -1
+    # (3 x 33 bins + 1 filter block) doubles for 64 x 64
+    assert lib.tdtm_prepare_image_workspace_bytes(64, 64, 0) == (3 * 33 + 1) * 8
+    assert lib.tdtm_prepare_image_workspace_bytes(64, 64, 100) == (3 * 100 + 1) * 8
+    fake = ctypes.c_void_p(16)  # never dereferenced: every check below fails first
+    assert lib.tdtm_rfft2(None, 64, 64, fake, None) != 0
+    assert b"null" in lib.tdtm_last_error()
+    assert lib.tdtm_rfft2(fake, 48, 64, fake, None) != 0
+    assert b"power of two" in lib.tdtm_last_error()
+    assert lib.tdtm_volume_rfft(fake, 1024, fake, None) != 0
+    assert b"power of two" in lib.tdtm_last_error()
+    assert lib.tdtm_prepare_image(fake, 64, 64, 31, 1, 0, 0.5, fake, fake, fake, None) != 0
+    assert b"template size" in lib.tdtm_last_error()
+    assert lib.tdtm_prepare_image(fake, 64, 64, 32, 1, 0, 0.0, fake, fake, fake, None) != 0
+    assert b"max_freq" in lib.tdtm_last_error()
+    params = (ctypes.c_double * 6)(300.0, 2.7, 1.5, 0.0, 1.0, 0.0)  # amplitude contrast >= 1
+    assert lib.tdtm_ctf_filter_stack(64, 1, 1, 1e4, 1e4, 0.0, fake, fake, params, fake, None) != 0
+    assert b"CTF" in lib.tdtm_last_error()
