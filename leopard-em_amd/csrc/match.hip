@@ -17,6 +17,7 @@
 // the row pass reads, whole 64-byte segments.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -35,6 +36,7 @@ constexpr int V = 8;   // values per thread in every FFT
 constexpr int CW = 2;  // columns per column-pass block (tile width)
 constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
 constexpr int TDTM_LDS_BYTES = 160 * 1024;  // LDS one workgroup may allocate on gfx950
+constexpr int REFINE_CHUNKS_MAX = 64;        // correlation chunks of a refinement row pass
 
 
 // ------------------------------------------------------------------------- kernels
@@ -736,8 +738,8 @@ __device__ __forceinline__ float2 w16_times(float2 a, int i) {
 }
 
 template <int WS, int MODE, int TWM, int RB, class Epi>
-__device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int K, int p, int r0,
-                                             const float2* __restrict__ tw_global, float* lds, Epi&& epi) {
+__device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int HH, int kbeg, int kend, int p,
+                                             int r0, const float2* __restrict__ tw_global, float* lds, Epi&& epi) {
   using C = RowCfg<WS, RB>;
   constexpr int M = C::M, T = C::T, LF = C::LF, MF = C::MF;
   constexpr bool ASYNC = MODE == 1;
@@ -768,9 +770,10 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
   const float2* gsrc = Gb + (long)p * C::TILEF + r0 * CW;
   float* raw2 = lds;                  // MODE 2: raw tile
   float* work2 = lds + 2 * C::TILE;   // MODE 2: de-interleaved regions / transform work area
-  if (MODE == 1 && K > 0) tile_block_async<WS>(gsrc, lds);
-  if (MODE == 2 && K > 0) tile_block_async<WS>(gsrc, raw2);
-  for (int k = 0; k < K; ++k) {
+  const int K = kend;
+  if (MODE == 1 && kbeg < K) tile_block_async<WS>(gsrc + kbeg * kstride, lds + (kbeg & 1) * C::BUF);
+  if (MODE == 2 && kbeg < K) tile_block_async<WS>(gsrc + kbeg * kstride, raw2);
+  for (int k = kbeg; k < K; ++k) {
     float* cur = MODE == 1 ? lds + (k & 1) * C::BUF : (MODE == 2 ? work2 : lds);
     float* nxt = lds + ((k + 1) & 1) * C::BUF;
     if (MODE == 1) {
@@ -889,7 +892,7 @@ __global__ __launch_bounds__(RB*(WS / 2) / V, 4) void k_rowpass(const float2* __
   }
 #pragma unroll
   for (int i = 0; i < V; ++i) ix2[i] = 0xffffffffu;
-  rowpass_loop<WS, MODE, TWM, RB>(Gb, HH, K, p, sub * RB, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, MODE, TWM, RB>(Gb, HH, 0, K, p, sub * RB, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const float a0 = v[i].x, a1 = v[i].y;
@@ -1025,19 +1028,25 @@ struct RefineBest {
   int k;
 };
 
-// better candidate: larger z; equal z -> smaller flat pixel index (torch.max, first)
-__device__ __forceinline__ bool ref_better(float z, int pix, float zb, int pixb) {
-  return z > zb || (z == zb && pix < pixb);
+// better candidate: larger z; equal z -> smaller flat pixel index (the first maximum of
+// the per-pixel maximum map); same pixel -> smaller correlation index (torch.max over
+// the correlation axis keeps the first maximum)
+__device__ __forceinline__ bool ref_better(float z, int pix, int k, float zb, int pixb, int kb) {
+  return z > zb || (z == zb && (pix < pixb || (pix == pixb && k < kb)));
 }
 
 // Row pass of a refinement batch: c2r along x for the rows of the valid region and the
 // z-score epilogue z = (cc - mean) / std on the valid crop.  Per pixel the first
 // correlation reaching the pixel's maximum z is kept (torch.max over the flattened
 // (pixel size, defocus, offset) axis); each block then reduces to its best
-// (z, pixel, correlation) and its maximum raw cc.
+// (z, pixel, correlation) and its maximum raw cc.  For a fixed pixel z is increasing in
+// cc (std > 0), so the loop tracks each pixel's maximum cc and its first correlation, and
+// z is evaluated once per pixel after the loop.  Block (tile row, chunk): the K
+// correlations are split into chunks of kchunk so that the grid fills the GPU even for
+// small crops (a 512 x 512 crop with a 256^3 template has 65 tile rows).
 template <int WS>
-__global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2* __restrict__ Gb, int HH, int G,
-                                                                   int K, int crop_h, int crop_w,
+__global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2* __restrict__ Gb, int HH, int K,
+                                                                   int kchunk, int crop_h, int crop_w,
                                                                    const float* __restrict__ mean,
                                                                    const float* __restrict__ stdv,
                                                                    const float2* __restrict__ tw,
@@ -1049,15 +1058,16 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   const int r = threadIdx.x / T, t = threadIdx.x % T;
   const int p = blockIdx.x;
   const int y = p * TR + r;
-  (void)G;
-  float zb[2 * V];
+  const int k0 = blockIdx.y * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  float cb[2 * V];
   unsigned kb2[V];
   float ccmax = -INFINITY;
 #pragma unroll
-  for (int j = 0; j < 2 * V; ++j) zb[j] = -INFINITY;
+  for (int j = 0; j < 2 * V; ++j) cb[j] = -INFINITY;
 #pragma unroll
   for (int i = 0; i < V; ++i) kb2[i] = 0xffffffffu;
-  rowpass_loop<WS, 3, 1, TR>(Gb, HH, K, p, 0, tw, lds, [&](int k, float2 (&v)[V]) {
+  rowpass_loop<WS, 3, 1, TR>(Gb, HH, k0, k1, p, 0, tw, lds, [&](int k, float2 (&v)[V]) {
 #pragma unroll
     for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -1066,16 +1076,14 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
         if (y < crop_h && x < crop_w) {
           const float c = hh ? v[i].y : v[i].x;
           ccmax = fmaxf(ccmax, c);
-          const long e = (long)y * crop_w + x;
-          const float z = (c - mean[e]) / stdv[e];
-          if (z > zb[2 * i + hh]) {
-            zb[2 * i + hh] = z;
+          if (c > cb[2 * i + hh]) {
+            cb[2 * i + hh] = c;
             kb2[i] = hh ? ((kb2[i] & 0x0000ffffu) | ((unsigned)k << 16)) : ((kb2[i] & 0xffff0000u) | (unsigned)k);
           }
         }
       }
   });
-  RefineBest b{-INFINITY, ccmax, 0x7fffffff, -1};
+  RefineBest b{-INFINITY, ccmax, 0x7fffffff, 0x7fffffff};
 #pragma unroll
   for (int i = 0; i < V; ++i)
 #pragma unroll
@@ -1084,8 +1092,9 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
       const unsigned kk = (kb2[i] >> (16 * hh)) & 0xffffu;
       if (y < crop_h && x < crop_w && kk != 0xffffu) {
         const int pix = y * crop_w + x;
-        if (ref_better(zb[2 * i + hh], pix, b.z, b.pix)) {
-          b.z = zb[2 * i + hh];
+        const float z = (cb[2 * i + hh] - mean[pix]) / stdv[pix];
+        if (ref_better(z, pix, (int)kk, b.z, b.pix, b.k)) {
+          b.z = z;
           b.pix = pix;
           b.k = (int)kk;
         }
@@ -1098,7 +1107,7 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
       RefineBest o = red[threadIdx.x + off];
       RefineBest sb = red[threadIdx.x];
       sb.cc = fmaxf(sb.cc, o.cc);
-      if (ref_better(o.z, o.pix, sb.z, sb.pix)) {
+      if (ref_better(o.z, o.pix, o.k, sb.z, sb.pix, sb.k)) {
         sb.z = o.z;
         sb.pix = o.pix;
         sb.k = o.k;
@@ -1107,17 +1116,17 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) part[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 // Reduce the row-pass blocks of one batch into its slot.
 __global__ void k_refine_reduce(const RefineBest* __restrict__ part, int n, RefineBest* __restrict__ slot) {
   __shared__ RefineBest red[256];
-  RefineBest b{-INFINITY, -INFINITY, 0x7fffffff, -1};
+  RefineBest b{-INFINITY, -INFINITY, 0x7fffffff, 0x7fffffff};
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     RefineBest o = part[i];
     b.cc = fmaxf(b.cc, o.cc);
-    if (ref_better(o.z, o.pix, b.z, b.pix)) {
+    if (ref_better(o.z, o.pix, o.k, b.z, b.pix, b.k)) {
       b.z = o.z;
       b.pix = o.pix;
       b.k = o.k;
@@ -1130,7 +1139,7 @@ __global__ void k_refine_reduce(const RefineBest* __restrict__ part, int n, Refi
       RefineBest o = red[threadIdx.x + off];
       RefineBest s = red[threadIdx.x];
       s.cc = fmaxf(s.cc, o.cc);
-      if (ref_better(o.z, o.pix, s.z, s.pix)) {
+      if (ref_better(o.z, o.pix, o.k, s.z, s.pix, s.k)) {
         s.z = o.z;
         s.pix = o.pix;
         s.k = o.k;
@@ -1464,10 +1473,10 @@ static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, flo
 }
 
 template <int WS>
-static void launch_rowpass_refine(tdtm_match_plan* p, int K, int crop_h, int crop_w, const float* mean,
+static void launch_rowpass_refine(tdtm_match_plan* p, int K, int kchunk, int crop_h, int crop_w, const float* mean,
                                   const float* sd, hipStream_t st) {
-  dim3 grid((crop_h + TR - 1) / TR);
-  k_rowpass_refine<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, p->G, K, crop_h, crop_w, mean, sd, p->tw,
+  const dim3 grid((crop_h + TR - 1) / TR, (K + kchunk - 1) / kchunk);
+  k_rowpass_refine<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, K, kchunk, crop_h, crop_w, mean, sd, p->tw,
                                                            p->rpart);
 }
 
@@ -1538,7 +1547,7 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   rc |= rc ? 0 : dalloc(&p->Rb, K * p->G * h * CW, p->bytes);
   rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->Gs * CW, p->bytes);
   rc |= rc ? 0 : dalloc(&p->ctfbuf, (size_t)C * D * h * p->wr, p->bytes);
-  rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR * REFINE_CHUNKS_MAX, p->bytes);
   if (rc) {
     plan_free(p);
     delete p;
@@ -1781,9 +1790,13 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
     TDTM_LAUNCHED();
     TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, ystore, st);
     TDTM_LAUNCHED();
-    TDTM_SWITCH_POW2(p->W, launch_rowpass_refine, p, K, crop_h, crop_w, corr_mean, corr_std, st);
+    // correlation chunks: about 8 blocks per CU over the tile rows of the crop
+    int nchunk = std::max(1, std::min({(8 * p->num_cu + nparts - 1) / nparts, K, REFINE_CHUNKS_MAX}));
+    const int kchunk = (K + nchunk - 1) / nchunk;
+    nchunk = (K + kchunk - 1) / kchunk;
+    TDTM_SWITCH_POW2(p->W, launch_rowpass_refine, p, K, kchunk, crop_h, crop_w, corr_mean, corr_std, st);
     TDTM_LAUNCHED();
-    k_refine_reduce<<<1, 256, 0, st>>>(p->rpart, nparts, p->rslots + bi);
+    k_refine_reduce<<<1, 256, 0, st>>>(p->rpart, nparts * nchunk, p->rslots + bi);
     TDTM_LAUNCHED();
   }
   k_refine_select<<<1, 64, 0, st>>>(p->rslots, nbatch, p->B, n_off, p->D, offsets, euler_pose, defocus_offsets,
