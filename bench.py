@@ -117,6 +117,67 @@ def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
     return n / dt, cores, n, dt
 
 
+def bench_refine(args) -> dict:
+    """Refinement throughput: ``--refine-particles`` 512 x 512 particle crops against a
+    256^3 template, a shared local grid of 17^3 orientation offsets (+/-2 deg at 0.25 deg)
+    x 13 defocus offsets (+/-300 A at 50 A) x 5 pixel-size offsets, batches of 32
+    offsets.  Inputs are device-resident random spectra (timing-equivalent); one warm-up
+    particle, then all particles timed between HIP events."""
+    import torch
+
+    from leopard_em_amd import core_refine_template
+
+    dev = torch.device("cuda:0")
+    n, H, h = args.refine_particles, 512, 256
+    g = torch.Generator(device=dev).manual_seed(0)
+    x = torch.randn((n, H, H // 2 + 1), dtype=torch.complex64, device=dev, generator=g)
+    tdft = torch.randn((h, h, h // 2 + 1), dtype=torch.complex64, device=dev, generator=g)
+    eul = torch.rand((n, 3), device=dev, generator=g) * torch.tensor([360.0, 180.0, 360.0], device=dev)
+    a = torch.arange(-2.0, 2.0001, 0.25)
+    offs = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1).reshape(-1, 3)
+    dfo = torch.arange(-300.0, 300.1, 50.0)
+    pxo = torch.linspace(-0.02, 0.02, 5)
+    crop = (H - h + 1, H - h + 1)
+    mean = torch.zeros((n,) + crop, device=dev)
+    std = torch.ones((n,) + crop, device=dev)
+    filt = torch.rand((n, h, h // 2 + 1), device=dev, generator=g)
+    du = torch.full((n,), 12000.0)
+    dv = torch.full((n,), 11800.0)
+    da = torch.full((n,), 30.0)
+    kw = dict(voltage=300.0, spherical_aberration=2.7, amplitude_contrast_ratio=0.07, phase_shift=0.0,
+              pixel_size=1.0, ctf_B_factor=0.0)
+
+    def run(sl):
+        return core_refine_template(x[sl], tdft, eul[sl], offs, dfo, du[sl], dv[sl], da[sl], pxo, mean[sl], std[sl],
+                                    kw, filt[sl], dev, batch_size=32)
+
+    run(slice(0, 1))
+    torch.cuda.synchronize(dev)
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    run(slice(0, n))
+    e1.record(stream)
+    e1.synchronize()
+    sec = e0.elapsed_time(e1) / 1e3
+    ncorr = n * offs.shape[0] * dfo.numel() * pxo.numel()
+    print(f"[bench] refine {n} particles: {sec:.2f} s, {ncorr / sec:.0f} corr/s", file=sys.stderr, flush=True)
+    return {
+        "metric": "refinement correlations/s (512^2 crops x 256^3 template)",
+        "value": ncorr / sec,
+        "unit": "correlations/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (random particle spectra, template spectrum and filters)",
+        "config": {"workload": f"core_refine_template, {n} particles of {H}x{H}, {h}^3 template, "
+                               f"{offs.shape[0]} orientation offsets x {dfo.numel()} defocus x {pxo.numel()} "
+                               "pixel sizes, batches of 32 offsets",
+                   "particles_per_s": n / sec},
+    }
+
+
 def read_traffic(path: str):
     """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     try:
@@ -146,12 +207,18 @@ def main():
     ap.add_argument("--no-phase-timing", action="store_true")
     ap.add_argument("--no-prep-timing", action="store_true",
                     help="skip timing the input preparation kernels (synthetic inputs only)")
+    ap.add_argument("--workload", choices=("match", "refine"), default="match",
+                    help="refine: particle refinement throughput instead of the search")
+    ap.add_argument("--refine-particles", type=int, default=3)
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (got --gpus {args.gpus}); multi-GPU sharding is a design note, "
               "not a measured configuration", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "refine":
+        print(json.dumps(bench_refine(args)), flush=True)
+        return
 
     import torch
 

@@ -272,16 +272,28 @@ __global__ void k_proj_norm(const float* __restrict__ part, int hs, int H, int W
   }
 }
 
+// Rows per row-forward block: 4 for the long rows (1024 threads at W = 4096), up to 16
+// for short rows so that small images (refinement crops) do not launch one small block
+// per 4 rows of every correlation.
+template <int WS>
+struct RowFwdCfg {
+  static constexpr int T = (WS / 2) / V;
+  static constexpr int RR = T >= 256 ? 4 : (T >= 128 ? 8 : 16);
+  static constexpr int BLOCK = RR * T;
+};
+
 // Zero-padded r2c along x (WS points, HS non-zero) of the normalised projection rows.
-// A block transforms RR = 4 consecutive rows at once (one row per T threads), so every
-// tile column gets 64 contiguous bytes from one block; the two blocks of each 8-row
-// group (a 128-byte line of Rb per tile column) get ids 8 apart, so they run on the same
-// XCD (workgroups go to the 8 XCDs round robin) and complete the lines in its L2.
+// A block transforms RR consecutive rows at once (one row per T threads), so every
+// tile column gets RR x 16 contiguous bytes from one block.  With RR = 4 the two blocks
+// of each 8-row group (a 128-byte line of Rb per tile column) get ids 8 apart, so they
+// run on the same XCD (workgroups go to the 8 XCDs round robin) and complete the lines
+// in its L2; with RR >= 8 a block writes whole lines.
 template <int WS, int NZJ>
-__global__ __launch_bounds__(4 * (WS / 2) / V) void k_rowfwd(const float* __restrict__ P, const float2* __restrict__ ae,
-                                                            int hs, int G, const float2* __restrict__ tw,
-                                                            float2* __restrict__ Rb) {
-  constexpr int RR = 4;
+__global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __restrict__ P,
+                                                                const float2* __restrict__ ae, int hs, int G,
+                                                                const float2* __restrict__ tw,
+                                                                float2* __restrict__ Rb) {
+  constexpr int RR = RowFwdCfg<WS>::RR;
   constexpr int M = WS / 2;
   constexpr int T = M / V;
   constexpr bool FOUR = UseFour<M>::value;
@@ -292,7 +304,7 @@ __global__ __launch_bounds__(4 * (WS / 2) / V) void k_rowfwd(const float* __rest
   float* lre = lds + r * 2 * LF;
   float* lim = lre + LF;
   int y0 = blockIdx.x * RR;
-  if (hs % 64 == 0) {
+  if (RR == 4 && hs % 64 == 0) {
     const int b = blockIdx.x, q = b >> 3;
     y0 = 8 * ((q >> 1) * 8 + (b & 7)) + RR * (q & 1);
   }
@@ -1338,8 +1350,9 @@ static int dispatch_proj(tdtm_match_plan* p, hipStream_t st) {
 
 template <int WS, int NZJ>
 static void launch_rowfwd_n(tdtm_match_plan* p, int K, hipStream_t st) {
-  dim3 grid(p->h / 4, K);  // 4 rows per block (h is a power of two >= 32)
-  k_rowfwd<WS, NZJ><<<grid, 4 * (WS / 2) / V, 0, st>>>(p->P, p->ae, p->h, p->G, p->tw, p->Rb);
+  using C = RowFwdCfg<WS>;
+  dim3 grid(p->h / C::RR, K);  // h is a power of two >= 32 >= RR
+  k_rowfwd<WS, NZJ><<<grid, C::BLOCK, 0, st>>>(p->P, p->ae, p->h, p->G, p->tw, p->Rb);
 }
 
 template <int WS>
