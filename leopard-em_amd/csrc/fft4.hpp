@@ -134,13 +134,42 @@ struct IsFsTw<FsTw<N>> {
   static constexpr bool value = true;
 };
 
+// The twiddles of the three twiddled stages of one lane, held in registers (w[1..7] of
+// stage 1, w[1..NW-1] of stage 2, w[1..7] of stage 3; w[0] = 1).  They depend on the
+// lane only, so a kernel that runs many transforms builds them once (fs_tw_regs) and
+// saves the per-transform loads and products; the inverse uses their conjugates.
+template <int N>
+struct FsTwReg {
+  float2 w1[7], w2[7], w3[7];
+};
+
+template <int N>
+__device__ __forceinline__ FsTwReg<N> opaque_offset(const FsTwReg<N>& t) {
+  return t;
+}
+
+template <class TWT>
+struct IsFsTwReg {
+  static constexpr bool value = false;
+};
+template <int N>
+struct IsFsTwReg<FsTwReg<N>> {
+  static constexpr bool value = true;
+};
+
 // Twiddles of four-step stage STAGE (1: W_N^{j t}, 2: W_S^{j l}, 3: W_64^{j p}) for lane
 // index k: w[j] for j < R, from a lane-ordered LDS copy or any indexable table.
 template <int STAGE, int N, int R, bool INV, class TWT>
 __device__ __forceinline__ void fs_tw(float2 (&w)[R], TWT tw, int k) {
   using L = FsTwLayout<N>;
   constexpr int step = STAGE == 1 ? TW_N / N : STAGE == 2 ? TW_N / L::S : TW_N / 64;
-  if constexpr (IsFsTw<TWT>::value) {
+  if constexpr (IsFsTwReg<TWT>::value) {
+    (void)k;
+    const float2* src = STAGE == 1 ? tw.w1 : STAGE == 2 ? tw.w2 : tw.w3;
+    w[0] = make_float2(1.f, 0.f);
+#pragma unroll
+    for (int j = 1; j < R; ++j) w[j] = INV ? c_conj(src[j - 1]) : src[j - 1];
+  } else if constexpr (IsFsTw<TWT>::value) {
     static_assert(STAGE != 2 || (R == L::NW), "stage 2 is radix NW");
     constexpr int off = STAGE == 1 ? L::O1 : STAGE == 2 ? L::O2 : L::O3;
     constexpr int cnt = STAGE == 1 ? L::T : STAGE == 2 ? 64 : 8;
@@ -163,6 +192,27 @@ __device__ __forceinline__ void fs_tw(float2 (&w)[R], TWT tw, int k) {
   } else {
     twiddles<R, INV>(w, tw, k, step);
   }
+}
+
+// Registers of lane t from any table fs_tw reads (the same loads and products, so the
+// transforms round identically with either form).
+template <int N, class TWT>
+__device__ __forceinline__ FsTwReg<N> fs_tw_regs(TWT tw, int t) {
+  constexpr int NW = FS<N>::NW;
+  const int l = t & 63, p = l & 7;
+  FsTwReg<N> r;
+  float2 w[8];
+  fs_tw<1, N, 8, false>(w, tw, t);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) r.w1[j - 1] = w[j];
+  float2 w2[NW];
+  fs_tw<2, N, NW, false>(w2, tw, l);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) r.w2[j - 1] = j < NW ? w2[j < NW ? j : 0] : make_float2(1.f, 0.f);
+  fs_tw<3, N, 8, false>(w, tw, p);
+#pragma unroll
+  for (int j = 1; j < 8; ++j) r.w3[j - 1] = w[j];
+  return r;
 }
 
 template <int R, int NT>

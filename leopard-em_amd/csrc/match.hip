@@ -443,7 +443,8 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // Sizes >= 512 use the four-step transform with NT correlations per iteration; smaller
 // sizes use the Stockham passes.  CWB = columns per block (1 or the CW columns of a
 // tile).  TWM = 1 reads the twiddles from an LDS copy of the table entries the
-// four-step transforms of size HH use; FULL = every row is stored (ystore == HH).
+// four-step transforms of size HH use; TWM = 2 builds each lane's twiddles once, in
+// registers, for all the block's correlations; FULL = every row is stored (ystore == HH).
 template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL, bool TS = false>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int Gs, int K, int kchunk, int ystore,
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   static_assert(TWM == 0 || FOUR, "the LDS twiddle table serves the four-step path");
   // TWM == 1: lane-ordered copy of the four-step twiddle bases (FsTwLayout) after the
   // transforms' work areas
-  __shared__ float lds[CWB * NT * 2 * LF + (TWM ? FsTwLayout<HF>::floats : 0)];
+  __shared__ float lds[CWB * NT * 2 * LF + (TWM == 1 ? FsTwLayout<HF>::floats : 0)];
   // block -> (tile column g, column c): the blocks that write the same 128-byte lines of
   // Gb -- tile columns 2m and 2m+1 (and both columns of each with one column per block)
   // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
@@ -480,15 +481,17 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
     fs_tw_fill<HF>(ltw, tw_global, threadIdx.x, CWB * T);
     __syncthreads();
   }
+  // four-step: the waves of one column are contiguous; Stockham: columns interleaved
+  const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
+  const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
   const auto tw = [&]() {
-    if constexpr (TWM == 1)
+    if constexpr (TWM == 2)
+      return fs_tw_regs<HF>(tw_global, t);
+    else if constexpr (TWM == 1)
       return FsTw<HF>{ltw};
     else
       return tw_global;
   }();
-  // four-step: the waves of one column are contiguous; Stockham: columns interleaved
-  const int cl = FOUR ? threadIdx.x / T : threadIdx.x % CW;  // column within the block
-  const int t = FOUR ? threadIdx.x % T : threadIdx.x / CW;
   if constexpr (CWB == CW) c = cl;
   const int k0 = blockIdx.y * kchunk;
   const int k1 = min(K, k0 + kchunk);
@@ -1271,7 +1274,7 @@ struct tdtm_match_plan {
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
-  int opt_colpass_twiddles = 1; // four-step column transforms: 0 global table, 1 LDS copy
+  int opt_colpass_twiddles = 2; // four-step column transforms: 0 global table, 1 LDS copy, 2 registers
   int opt_colpass_store = 0;    // two-column blocks: 0 direct 8-byte stores, 1 whole rows via LDS
   int opt_rowpass_staging = 3;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat,
                                 // 3: synchronous de-interleaved with batched loads
@@ -1377,6 +1380,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   constexpr int TWF = FsTwLayout<(FOUR ? HH : 512)>::floats;
   const bool full = ystore == HH;
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
+  const bool rtw = FOUR && p->opt_colpass_twiddles == 2;
   using Kern = void (*)(const float2*, const float2*, int, int, int, int, int, int, const float2*, float2*);
   Kern kern = nullptr;
 #define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                       \
@@ -1391,7 +1395,11 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
-    if (ltw && fits_) {                                                                   \
+    constexpr bool regs_ = FOUR && CWB_ == CW && NT_ == 1;                                \
+    if (rtw && regs_) {                                                                   \
+      if (full) TDTM_COLPASS_T(CWB_, NT_, (regs_ ? 2 : 0), true);                         \
+      else TDTM_COLPASS_T(CWB_, NT_, (regs_ ? 2 : 0), false);                             \
+    } else if ((ltw || rtw) && fits_) {                                                   \
       if (full) TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), true);                         \
       else TDTM_COLPASS_T(CWB_, NT_, (fits_ ? 1 : 0), false);                             \
     } else {                                                                              \
@@ -1598,7 +1606,7 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     if (value < 0 || value > 1) return fail("colpass_store must be 0 or 1");
     p->opt_colpass_store = value;
   } else if (k == "colpass_twiddles") {
-    if (value < 0 || value > 1) return fail("colpass_twiddles must be 0 or 1");
+    if (value < 0 || value > 2) return fail("colpass_twiddles must be 0, 1 or 2");
     p->opt_colpass_twiddles = value;
   } else if (k == "colpass_chunks") {
     if (value < 0) return fail("colpass_chunks must be >= 0");

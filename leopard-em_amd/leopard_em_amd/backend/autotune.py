@@ -26,6 +26,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "leopard_em_amd"
 CANDIDATES: list[dict[str, int]] = [
     {},
     {"colpass_pair": 1},
+    {"colpass_twiddles": 1},
     {"colpass_twiddles": 0},
     {"colpass_columns": 1},
     {"colpass_columns": 1, "colpass_pair": 0},
@@ -40,7 +41,7 @@ _OPTION_DEFAULTS = {
     "colpass_columns": 2,
     "colpass_pair": 0,
     "colpass_chunks": 0,
-    "colpass_twiddles": 1,
+    "colpass_twiddles": 2,
     "colpass_store": 0,
     "rowpass_twiddles": 1,
     "rowpass_staging": 3,

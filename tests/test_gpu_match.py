@@ -157,6 +157,8 @@ def test_match_peak_list_identical(cuda_device):
     {"colpass_twiddles": 0}, {"colpass_twiddles": 0, "colpass_pair": 1}, {"colpass_columns": 1, "colpass_chunks": 5},
     {"rowpass_rows": 2}, {"rowpass_rows": 1}, {"rowpass_rows": 1, "rowpass_twiddles": 0, "rowpass_staging": 0},
     {"colpass_store": 1}, {"colpass_store": 1, "colpass_twiddles": 0},
+    {"colpass_twiddles": 2}, {"colpass_twiddles": 2, "colpass_store": 1}, {"colpass_twiddles": 2, "colpass_pair": 1},
+    {"colpass_twiddles": 2, "colpass_columns": 1}, {"colpass_twiddles": 1}, {"colpass_twiddles": 1, "colpass_store": 1},
     {"phase_timing": 1},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
@@ -174,6 +176,37 @@ def test_launch_options_do_not_change_results(opts, cuda_device):
         plan.run(tdft.to(cuda_device), eul.to(cuda_device), 0, 8)
         torch.cuda.synchronize()
         out.append(raw_statistics(plan))
+    for k in out[0]:
+        assert torch.equal(out[0][k], out[1][k]), k
+
+
+@pytest.mark.parametrize("opts", [{"colpass_twiddles": 2}, {"colpass_twiddles": 1}, {"colpass_store": 1}],
+                         ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
+def test_launch_options_do_not_change_results_4096(opts, cuda_device):
+    """The production image size (eight-wave four-step column transforms)."""
+    from leopard_em_amd.backend import MatchPlan, raw_statistics
+
+    print(f"options at 4096: {opts}", flush=True)
+    H = W = 4096
+    h, D = 64, 2
+    gen = torch.Generator().manual_seed(5)
+    img = torch.randn((H, W // 2 + 1), dtype=torch.complex64, generator=gen)
+    img[0, 0] = 0
+    tdft = torch.randn((h, h, h // 2 + 1), dtype=torch.complex64, generator=gen).to(cuda_device)
+    ctf = torch.rand((1, D, h, h // 2 + 1), generator=gen)
+    wf = torch.rand((h, h // 2 + 1), generator=gen)
+    eul = (torch.rand((4, 3), generator=gen) * torch.tensor([360.0, 180.0, 360.0])).to(cuda_device)
+    out = []
+    for o in ({"colpass_twiddles": 0}, opts):
+        plan = MatchPlan(H, W, h, 2, D, 1, cuda_device)
+        for k, v in o.items():
+            plan.set_option(k, v)
+        plan.set_inputs(img, ctf, wf)
+        plan.reset_stats()
+        plan.run(tdft, eul, 0, 4)
+        torch.cuda.synchronize()
+        out.append(raw_statistics(plan))
+        del plan
     for k in out[0]:
         assert torch.equal(out[0][k], out[1][k]), k
 
