@@ -338,19 +338,26 @@ __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __
     lim[p] = v[i].y;
   }
   __syncthreads();
+  // split into the W-point spectrum and store: thread -> (row r2 = tid % RR, tile column
+  // g), so that a wave's 16-byte stores (both columns of a tile column for one row) cover
+  // RR consecutive rows, i.e. RR x 16 contiguous bytes per tile column
   float2* out = Rb + (long)k * G * hs * CW;
-  for (int kx = t; kx < G * CW; kx += T) {
-    float2 X = make_float2(0.f, 0.f);
-    if (kx <= M) {
-      const int a = kx & (M - 1), bi = (M - kx) & (M - 1);
-      const float2 A = make_float2(lre[lds_pad(a)], lim[lds_pad(a)]);
-      const float2 Bc = make_float2(lre[lds_pad(bi)], -lim[lds_pad(bi)]);
-      const float2 E = c_scale(c_add(A, Bc), 0.5f);
-      const float2 O = c_scale(c_sub(A, Bc), 0.5f);
-      const float2 wo = c_mul(tw[kx * (TW_N / WS)], O);  // e^{-2 pi i kx/WS} * O
-      X = make_float2(E.x + wo.y, E.y - wo.x);           // E - i*wo
-    }
-    out[((long)(kx / CW) * hs + y) * CW + (kx % CW)] = X;
+  const int r2 = threadIdx.x % RR;
+  const float* sre = lds + r2 * 2 * LF;
+  const float* sim = sre + LF;
+  auto split = [&](int kx) {
+    if (kx > M) return make_float2(0.f, 0.f);
+    const int a = kx & (M - 1), bi = (M - kx) & (M - 1);
+    const float2 A = make_float2(sre[lds_pad(a)], sim[lds_pad(a)]);
+    const float2 Bc = make_float2(sre[lds_pad(bi)], -sim[lds_pad(bi)]);
+    const float2 E = c_scale(c_add(A, Bc), 0.5f);
+    const float2 O = c_scale(c_sub(A, Bc), 0.5f);
+    const float2 wo = c_mul(tw[kx * (TW_N / WS)], O);  // e^{-2 pi i kx/WS} * O
+    return make_float2(E.x + wo.y, E.y - wo.x);         // E - i*wo
+  };
+  for (int g = threadIdx.x / RR; g < G; g += RowFwdCfg<WS>::BLOCK / RR) {
+    const float2 X0 = split(CW * g), X1 = split(CW * g + 1);
+    *reinterpret_cast<float4*>(out + ((long)g * hs + y0 + r2) * CW) = make_float4(X0.x, X0.y, X1.x, X1.y);
   }
 }
 
