@@ -42,14 +42,26 @@ const char* tdtm_last_error(void);
  * an h^3 template and up to max_orientation_batch orientations x num_defocus x
  * num_pixel correlations per step.  Replaces the per-device set-up inside
  * _core_match_template_single_gpu (src/leopard_em/backend/core_match_template.py:369-486).
- * Supported sizes: H, W in {64 .. 4096} powers of two; h in {32 .. 512} powers of two,
- * h <= min(H, W).
+ * Supported sizes: H in {64 .. 4096}, W in {64 .. 8192}, powers of two; h in {32 .. 512}
+ * powers of two, h <= min(H, W).  Other image sizes run in the next power-of-two grid
+ * (tdtm_match_plan_set_image_size, tdtm_irfft2_any).
  */
 int tdtm_match_plan_create(int image_h, int image_w, int template_h, int max_orientation_batch,
                            int num_defocus, int num_pixel, tdtm_match_plan_t* out_plan);
 int tdtm_match_plan_destroy(tdtm_match_plan_t plan);
 /* Device bytes held by the plan. */
 int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
+
+/*
+ * Image inside a larger FFT grid: the plan's H x W grid holds an image_h x image_w
+ * micrograph zero-padded at the bottom / right (sizes that are not powers of two, e.g.
+ * 4092 x 5760 in a 4096 x 8192 grid).  Sets the image size the projection normalisation
+ * uses (normalize_template_projection's relative size and padded variance,
+ * src/leopard_em/backend/utils.py:106-188); correlations at positions whose template
+ * footprint lies inside the image are then those of the unpadded search.  Default: the
+ * grid size.  No counterpart in the reference (torch.fft handles any size there).
+ */
+int tdtm_match_plan_set_image_size(tdtm_match_plan_t plan, int image_h, int image_w);
 
 /*
  * Launch-shape options of a plan (no counterpart in the reference; the autotuner in
@@ -145,12 +157,15 @@ int tdtm_stats_merge(int64_t n, float* mip_a, int32_t* idx_a, float* sum_hi_a, f
  * (src/leopard_em/backend/core_refine_template.py:555-767) including its per-particle
  * CTF stack (utils/ctf_utils.py:19-116) and the angle composition / wrap of
  * core_refine_template.py:504-536.  The plan is created with the crop size as image
- * size, num_defocus = len(defocus_offsets), num_pixel = len(pixel_offsets).
- *   particle_dft      complex64 (H, W/2+1), euler_pose float32 (3) on the device,
+ * size (or, for crop sizes that are not powers of two, with the next power-of-two grid
+ * and tdtm_match_plan_set_image_size(crop size); particle_dft is then the spectrum of the
+ * zero-padded crop, e.g. from tdtm_irfft2_any + tdtm_rfft2), num_defocus =
+ * len(defocus_offsets), num_pixel = len(pixel_offsets).
+ *   particle_dft      complex64 (grid H, grid W/2+1), euler_pose float32 (3) on the device,
  *   offsets           float32 (n_off, 3) degrees,  defocus_offsets (D), pixel_offsets (C),
  *   ctf_params        host doubles: voltage kV, Cs mm, amplitude contrast, phase shift
  *                     deg, pixel size A, B-factor A^2,
- *   projective_filter float32 (h, h/2+1),  corr_mean / corr_std float32 (H-h+1, W-h+1),
+ *   projective_filter float32 (h, h/2+1),  corr_mean / corr_std float32 (crop H-h+1, crop W-h+1),
  *   out               float32[14] on the device: max_cc, max_z, phi, theta, psi (composed,
  *                     wrapped), defocus offset, pixel-size offset, pos_y, pos_x (valid-crop
  *                     coordinates), angle_idx, unused, phi/theta/psi offsets.
@@ -170,8 +185,8 @@ int tdtm_refine_particle(tdtm_match_plan_t plan, const void* template_dft, const
 
 /*
  * Unnormalised forward 2D real FFT: image float32 (H, W) -> out complex64 (H, W/2+1),
- * as torch.fft.rfftn(image) (match_template_manager.py:160).  H, W powers of two in
- * [32, 4096].
+ * as torch.fft.rfftn(image) (match_template_manager.py:160).  H, W powers of two,
+ * H in [32, 4096], W in [32, 8192].
  */
 int tdtm_rfft2(const float* image, int H, int W, void* out, void* stream);
 
@@ -181,6 +196,19 @@ int tdtm_rfft2(const float* image, int H, int W, void* out, void* stream);
  * (src/leopard_em/utils/image_processing.py:179-202).  n a power of two in [32, 512].
  */
 int tdtm_volume_rfft(const float* volume, int n, void* out, void* stream);
+
+/*
+ * Inverse 2D real DFT of any size (direct sums in double): spectrum complex64 (H, W/2+1)
+ * -> image float32 rows 0 .. H-1, columns 0 .. W-1 of a row-major array with row stride
+ * ld (>= W) floats, with torch.fft.irfftn's conventions (1/(H W) normalisation, the
+ * imaginary parts of DC and Nyquist ignored).  The other elements of `image` are not
+ * written, so a zero-filled array receives the image zero-padded.  Used to move a
+ * spectrum whose size is not a power of two onto the padded grid of a plan (once per
+ * micrograph).  1 <= H <= 4096, W even <= 8192; workspace of
+ * tdtm_irfft2_any_workspace_bytes(H, W) device bytes.  No counterpart in the reference.
+ */
+int64_t tdtm_irfft2_any_workspace_bytes(int H, int W);
+int tdtm_irfft2_any(const void* spectrum, int H, int W, float* image, int64_t ld, void* workspace, void* stream);
 
 /* Device workspace bytes tdtm_prepare_image needs (-1: unsupported size or bin count). */
 int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins);

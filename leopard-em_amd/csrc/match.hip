@@ -278,16 +278,16 @@ __global__ void k_proj_norm(const float* __restrict__ part, int hs, int H, int W
 template <int WS>
 struct RowFwdCfg {
   static constexpr int T = (WS / 2) / V;
-  static constexpr int RR = T >= 256 ? 4 : (T >= 128 ? 8 : 16);
+  static constexpr int RR = T >= 512 ? 2 : (T >= 256 ? 4 : (T >= 128 ? 8 : 16));
   static constexpr int BLOCK = RR * T;
 };
 
 // Zero-padded r2c along x (WS points, HS non-zero) of the normalised projection rows.
 // A block transforms RR consecutive rows at once (one row per T threads), so every
-// tile column gets RR x 16 contiguous bytes from one block.  With RR = 4 the two blocks
-// of each 8-row group (a 128-byte line of Rb per tile column) get ids 8 apart, so they
-// run on the same XCD (workgroups go to the 8 XCDs round robin) and complete the lines
-// in its L2; with RR >= 8 a block writes whole lines.
+// tile column gets RR x 16 contiguous bytes from one block.  With RR < 8 the 8 / RR
+// blocks of each 8-row group (a 128-byte line of Rb per tile column) get ids 8 apart, so
+// they run on the same XCD (workgroups go to the 8 XCDs round robin) and complete the
+// lines in its L2; with RR >= 8 a block writes whole lines.
 template <int WS, int NZJ>
 __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __restrict__ P,
                                                                 const float2* __restrict__ ae, int hs, int G,
@@ -304,9 +304,11 @@ __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __
   float* lre = lds + r * 2 * LF;
   float* lim = lre + LF;
   int y0 = blockIdx.x * RR;
-  if (RR == 4 && hs % 64 == 0) {
+  if (RR < 8 && hs % 64 == 0) {
+    // the 8 / RR blocks of an 8-row line get ids 8 apart (one XCD)
+    constexpr int NB = 8 / RR;
     const int b = blockIdx.x, q = b >> 3;
-    y0 = 8 * ((q >> 1) * 8 + (b & 7)) + RR * (q & 1);
+    y0 = 8 * ((q / NB) * 8 + (b & 7)) + RR * (q % NB);
   }
   const int y = y0 + r;
   const int k = blockIdx.y;
@@ -1275,6 +1277,8 @@ using namespace tdtm;
 
 struct tdtm_match_plan {
   int H, W, h, B, D, C, K, G, Wr, wr;
+  int Hn, Wn;  // image size of the projection normalisation (the FFT grid H x W, or the
+               // unpadded image inside it: tdtm_match_plan_set_image_size)
   int Gs;  // tile-column stride of the intermediate (G rounded up to even: 128-byte lines)
   int num_cu = 256;  // compute units of the plan's device
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
@@ -1478,14 +1482,34 @@ static void launch_rowpass_mt(tdtm_match_plan* p, int K, int nb, int O, int o0, 
                               float* slo, float* qhi, float* qlo, hipStream_t st) {
   if constexpr (MODE == 0 || MODE == 3) {
     if (p->opt_rowpass_rows == 1) return launch_rowpass_rb<WS, MODE, TWM, 1>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
-    if (p->opt_rowpass_rows == 2) return launch_rowpass_rb<WS, MODE, TWM, 2>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+    if (p->opt_rowpass_rows == 2 || WS > 4096)
+      return launch_rowpass_rb<WS, MODE, TWM, 2>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
   }
-  launch_rowpass_rb<WS, MODE, TWM, TR>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+  if constexpr (WS <= 4096) launch_rowpass_rb<WS, MODE, TWM, TR>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
 }
+
+template <int WS>
+static void launch_rowpass_pow2(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
+                                float* slo, float* qhi, float* qlo, hipStream_t st);
 
 template <int WS>
 static void launch_rowpass(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                            float* slo, float* qhi, float* qlo, hipStream_t st) {
+  if constexpr (WS > 4096) {
+    // 8192-sample rows: one or two rows per block, synchronous staging (the other shapes
+    // do not fit the LDS); rowpass_rows = 4 runs as 2
+    if (p->opt_rowpass_twiddles == 1)
+      launch_rowpass_mt<WS, 3, 1>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+    else
+      launch_rowpass_mt<WS, 3, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+  } else {
+    launch_rowpass_pow2<WS>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st);
+  }
+}
+
+template <int WS>
+static void launch_rowpass_pow2(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
+                                float* slo, float* qhi, float* qlo, hipStream_t st) {
   const bool ltw = p->opt_rowpass_twiddles == 1;
   switch (p->opt_rowpass_staging) {
     case 1: launch_rowpass_mt<WS, 1, 0>(p, K, nb, O, o0, mip, idx, shi, slo, qhi, qlo, st); break;
@@ -1520,6 +1544,20 @@ static void launch_rowpass_refine(tdtm_match_plan* p, int K, int kchunk, int cro
     default: return fail("unsupported image size"); \
   }
 
+// row lengths of the search's row passes: up to 8192 (padded grids of wide micrographs)
+#define TDTM_SWITCH_ROWS(val, FN, ...)             \
+  switch (val) {                                   \
+    case 64: FN<64>(__VA_ARGS__); break;           \
+    case 128: FN<128>(__VA_ARGS__); break;         \
+    case 256: FN<256>(__VA_ARGS__); break;         \
+    case 512: FN<512>(__VA_ARGS__); break;         \
+    case 1024: FN<1024>(__VA_ARGS__); break;       \
+    case 2048: FN<2048>(__VA_ARGS__); break;       \
+    case 4096: FN<4096>(__VA_ARGS__); break;       \
+    case 8192: FN<8192>(__VA_ARGS__); break;       \
+    default: return fail("unsupported image size"); \
+  }
+
 extern "C" {
 
 int tdtm_abi_version(void) { return 1; }
@@ -1538,8 +1576,8 @@ const char* tdtm_last_error(void) { return last_error().c_str(); }
 int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_plan_t* out) {
   if (!out) return fail("out_plan is NULL");
   *out = nullptr;
-  if (!is_pow2(H) || !is_pow2(W) || H < 64 || W < 64 || H > 4096 || W > 4096)
-    return fail("image size must be a power of two in [64, 4096], got " + std::to_string(H) + "x" +
+  if (!is_pow2(H) || !is_pow2(W) || H < 64 || W < 64 || H > 4096 || W > 8192)
+    return fail("image size must be a power of two in [64, 4096] x [64, 8192], got " + std::to_string(H) + "x" +
                 std::to_string(W));
   if (!is_pow2(h) || h < 32 || h > 512) return fail("template size must be a power of two in [32, 512]");
   if (h > H || h > W) return fail("template larger than image");
@@ -1553,6 +1591,8 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   }
   p->H = H;
   p->W = W;
+  p->Hn = H;
+  p->Wn = W;
   p->h = h;
   p->B = B;
   p->D = D;
@@ -1600,6 +1640,16 @@ int tdtm_match_plan_destroy(tdtm_match_plan_t p) {
 }
 
 int64_t tdtm_match_plan_bytes(tdtm_match_plan_t p) { return p ? p->bytes : -1; }
+
+int tdtm_match_plan_set_image_size(tdtm_match_plan_t p, int image_h, int image_w) {
+  if (!p) return fail("null argument");
+  if (image_h < p->h || image_w < p->h || image_h > p->H || image_w > p->W)
+    return fail("image size must lie between the template size and the plan's grid " + std::to_string(p->H) + "x" +
+                std::to_string(p->W));
+  p->Hn = image_h;
+  p->Wn = image_w;
+  return 0;
+}
 
 int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) {
   if (!p || !key) return fail("null argument");
@@ -1713,16 +1763,16 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   p->K = Ksave;
   if (rc) return rc;
   TDTM_LAUNCHED();
-  k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->H, p->W, p->ae);
+  k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->Hn, p->Wn, p->ae);
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[1], st));
-  TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
+  TDTM_SWITCH_ROWS(p->W, launch_rowfwd, p, K, st);
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[2], st));
   TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, p->H, st);
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[3], st));
-  TDTM_SWITCH_POW2(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
+  TDTM_SWITCH_ROWS(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[4], st));
   return 0;
@@ -1777,7 +1827,9 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
   if (n_off < 1) return fail("no orientation offsets");
   if ((long)p->B * p->D * p->C >= 65535) return fail("batch x defocus x pixel sizes must be < 65535");
   hipStream_t st = (hipStream_t)stream;
-  const int crop_h = p->H - p->h + 1, crop_w = p->W - p->h + 1;
+  if (p->W > 4096) return fail("refinement crops are at most 4096 wide");
+  // valid region of the crop (the image inside a padded grid: tdtm_match_plan_set_image_size)
+  const int crop_h = p->Hn - p->h + 1, crop_w = p->Wn - p->h + 1;
   const int nbatch = (n_off + p->B - 1) / p->B;
   if (nbatch > p->rslots_cap) {
     if (p->rslots) TDTM_HIP(hipFree(p->rslots));
@@ -1812,7 +1864,7 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
     p->K = Ksave;
     if (rc) return rc;
     TDTM_LAUNCHED();
-    k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->H, p->W, p->ae);
+    k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->Hn, p->Wn, p->ae);
     TDTM_LAUNCHED();
     TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
     TDTM_LAUNCHED();

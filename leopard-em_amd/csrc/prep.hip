@@ -11,6 +11,9 @@
 //   k_whiten_eval  the curve interpolated onto an rfft grid (the whitening filter)
 //   k_prep_filter  image spectrum x whitening + partial sums of its energy
 //   k_prep_scale   normalisation to unit real-space variance
+//   k_idft_cols / k_idft_rows
+//                  inverse 2D real DFT of any size (direct sums in double), to move an
+//                  image spectrum whose size is not a power of two onto a padded grid
 //
 // Everything is HBM-bound streaming work over the (H, W/2+1) spectrum or the n^3 volume;
 // the per-bin and per-block reductions run in a fixed order, so results do not depend
@@ -336,6 +339,68 @@ __global__ __launch_bounds__(FILTER_BLOCK) void k_prep_scale(float2* __restrict_
   }
 }
 
+// Any-size inverse DFTs (once per micrograph, to re-grid a spectrum): direct sums in
+// double.  Each thread's rotation e^{2 pi i m / N} is advanced by complex products
+// (rounding drift ~N ulp of double, far below float precision).
+constexpr int DFT_BLOCK = 256;
+constexpr int DFT_MAX_H = 4096;  // column length (LDS: DFT_MAX_H double2)
+constexpr int DFT_MAX_W = 8192;  // row length (LDS: DFT_MAX_W / 2 + 1 double2)
+
+// Y[y][kx] = sum_ky X[ky][kx] e^{+2 pi i ky y / H} for every column kx < Wr of an (H, Wr)
+// complex64 spectrum; one block per column, the column staged in LDS.
+__global__ __launch_bounds__(DFT_BLOCK) void k_idft_cols(const float2* __restrict__ X, int H, int Wr,
+                                                        double2* __restrict__ Y) {
+  __shared__ double2 col[DFT_MAX_H];
+  const int kx = blockIdx.x;
+  for (int j = threadIdx.x; j < H; j += DFT_BLOCK) {
+    const float2 v = X[(long)j * Wr + kx];
+    col[j] = make_double2(v.x, v.y);
+  }
+  __syncthreads();
+  for (int y = threadIdx.x; y < H; y += DFT_BLOCK) {
+    double sn, cs;
+    sincospi(2.0 * (double)y / (double)H, &sn, &cs);
+    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    for (int ky = 0; ky < H; ++ky) {
+      const double2 c = col[ky];
+      ar = fma(c.x, wr, fma(-c.y, wi, ar));
+      ai = fma(c.x, wi, fma(c.y, wr, ai));
+      const double nr = fma(wr, cs, -wi * sn);
+      wi = fma(wr, sn, wi * cs);
+      wr = nr;
+    }
+    Y[(long)y * Wr + kx] = make_double2(ar, ai);
+  }
+}
+
+// x[y][n] = s (Re Y[0] + (-1)^n Re Y[W/2] + 2 sum_{k=1}^{W/2-1} Re(Y[k] e^{+2 pi i k n / W}))
+// for every row y < H (the imaginary parts of DC and Nyquist are ignored, as pocketfft's
+// c2r does; s = 1 / (H W), torch's default normalisation), written as float into row y
+// of a (rows, ld) array; one block per row, the row staged in LDS.
+__global__ __launch_bounds__(DFT_BLOCK) void k_idft_rows(const double2* __restrict__ Y, int H, int W,
+                                                        float* __restrict__ out, long ld) {
+  __shared__ double2 row[DFT_MAX_W / 2 + 1];
+  const int y = blockIdx.x;
+  const int M = W / 2, Wr = M + 1;
+  for (int j = threadIdx.x; j < Wr; j += DFT_BLOCK) row[j] = Y[(long)y * Wr + j];
+  __syncthreads();
+  const double scale = 1.0 / ((double)H * (double)W);
+  for (int n = threadIdx.x; n < W; n += DFT_BLOCK) {
+    double sn, cs;
+    sincospi(2.0 * (double)n / (double)W, &sn, &cs);
+    double wr = cs, wi = sn, acc = 0.0;  // k = 1
+    for (int k = 1; k < M; ++k) {
+      const double2 c = row[k];
+      acc = fma(c.x, wr, fma(-c.y, wi, acc));
+      const double nr = fma(wr, cs, -wi * sn);
+      wi = fma(wr, sn, wi * cs);
+      wr = nr;
+    }
+    const double v = row[0].x + ((n & 1) ? -row[M].x : row[M].x) + 2.0 * acc;
+    out[(long)y * ld + n] = (float)(v * scale);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int M, bool SHIFT>
 int launch_rows_m(const float* x, long rows, const float2* tw, float2* out, hipStream_t st) {
@@ -358,6 +423,7 @@ int launch_rows(int w, const float* x, long rows, const float2* tw, float2* out,
     case 1024: return launch_rows_m<512, SHIFT>(x, rows, tw, out, st);
     case 2048: return launch_rows_m<1024, SHIFT>(x, rows, tw, out, st);
     case 4096: return launch_rows_m<2048, SHIFT>(x, rows, tw, out, st);
+    case 8192: return launch_rows_m<4096, SHIFT>(x, rows, tw, out, st);
     default: return fail("unsupported row length");
   }
 }
@@ -429,8 +495,9 @@ extern "C" {
 
 int tdtm_rfft2(const float* image, int H, int W, void* out, void* stream) {
   if (!image || !out) return fail("null argument");
-  if (!is_pow2(H) || !is_pow2(W) || H < 32 || W < 32 || H > 4096 || W > 4096)
-    return fail("image size must be a power of two in [32, 4096], got " + std::to_string(H) + "x" + std::to_string(W));
+  if (!is_pow2(H) || !is_pow2(W) || H < 32 || W < 32 || H > 4096 || W > 8192)
+    return fail("image size must be a power of two in [32, 4096] x [32, 8192], got " + std::to_string(H) + "x" +
+                std::to_string(W));
   return rfft2_impl(image, H, W, static_cast<float2*>(out), static_cast<hipStream_t>(stream));
 }
 
@@ -447,6 +514,26 @@ int tdtm_volume_rfft(const float* volume, int n, void* out, void* stream) {
   if (launch_rows<true>(n, volume, (long)n * n, tw, o, st)) return 1;
   if (launch_cols<true>(n, o, wr, wr, (long)n * wr, n, tw, st)) return 1;
   return launch_cols<true>(n, o, n * wr, (long)n * wr, 0, 1, tw, st);
+}
+
+int64_t tdtm_irfft2_any_workspace_bytes(int H, int W) {
+  if (H < 1 || H > DFT_MAX_H || W < 2 || W > DFT_MAX_W || (W & 1)) return -1;
+  return (int64_t)H * (W / 2 + 1) * (int64_t)sizeof(double2);
+}
+
+int tdtm_irfft2_any(const void* spectrum, int H, int W, float* image, int64_t ld, void* workspace, void* stream) {
+  if (!spectrum || !image || !workspace) return fail("null argument");
+  if (H < 1 || H > DFT_MAX_H || W < 2 || W > DFT_MAX_W || (W & 1))
+    return fail("irfft2_any: need 1 <= H <= 4096 and an even W <= 8192, got " + std::to_string(H) + "x" +
+                std::to_string(W));
+  if (ld < W) return fail("irfft2_any: row stride below the image width");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  double2* Y = static_cast<double2*>(workspace);
+  k_idft_cols<<<W / 2 + 1, DFT_BLOCK, 0, st>>>(static_cast<const float2*>(spectrum), H, W / 2 + 1, Y);
+  TDTM_LAUNCHED();
+  k_idft_rows<<<H, DFT_BLOCK, 0, st>>>(Y, H, W, image, (long)ld);
+  TDTM_LAUNCHED();
+  return 0;
 }
 
 int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins) {

@@ -26,6 +26,7 @@ SIGNATURES = {
     "tdtm_match_plan_destroy": (c_int, [c_void_p]),
     "tdtm_match_plan_bytes": (c_int64, [c_void_p]),
     "tdtm_match_plan_set_option": (c_int, [c_void_p, c_char_p, c_int]),
+    "tdtm_match_plan_set_image_size": (c_int, [c_void_p, c_int, c_int]),
     "tdtm_match_plan_phase_times": (c_int, [c_void_p, c_void_p, c_void_p]),
     "tdtm_match_set_image": (c_int, [c_void_p, c_void_p, c_void_p]),
     "tdtm_match_set_filters": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -39,6 +40,8 @@ SIGNATURES = {
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
     "tdtm_rfft2": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tdtm_volume_rfft": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "tdtm_irfft2_any_workspace_bytes": (c_int64, [c_int, c_int]),
+    "tdtm_irfft2_any": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
     "tdtm_prepare_image_workspace_bytes": (c_int64, [c_int, c_int, c_int]),
     "tdtm_prepare_image": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.c_double, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),

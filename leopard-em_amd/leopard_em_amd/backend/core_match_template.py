@@ -17,6 +17,12 @@ output dictionary.  Differences, all deliberate:
   work of a step is one fused launch sequence on the current stream.
 * with the environment variable ``TDTM_AUTOTUNE=1`` each device's plan picks its
   launch shapes with ``backend.autotune`` (cached per shape and device).
+* image sizes that are not powers of two (e.g. 4092 x 5760 K3 micrographs) run in the
+  next power-of-two grid with the image zero-padded at the bottom / right: every
+  position whose template footprint lies inside the image (the valid region the peak
+  search uses) gets the reference's value; the positions near the bottom / right edges
+  whose footprint crosses the edge see zero padding instead of the reference's
+  periodic wrap-around.
 """
 
 from __future__ import annotations
@@ -34,9 +40,38 @@ from .. import _lib
 
 CHECKPOINT_FORMAT = "tdtm-match-stats-1"
 
+MAX_GRID_H, MAX_GRID_W = 4096, 8192
+
+
+def _pow2(n: int) -> bool:
+    return n > 0 and n & (n - 1) == 0
+
+
+def search_grid(image_h: int, image_w: int) -> tuple[int, int]:
+    """FFT grid of the search for an image size: the size itself when both edges are
+    powers of two the kernels support, else the next powers of two (at least 64)."""
+    if _pow2(image_h) and _pow2(image_w) and 64 <= image_h <= MAX_GRID_H and 64 <= image_w <= MAX_GRID_W:
+        return image_h, image_w
+
+    def up(n: int) -> int:
+        g = 64
+        while g < n:
+            g *= 2
+        return g
+
+    gh, gw = up(image_h), up(image_w)
+    if gh > MAX_GRID_H or gw > MAX_GRID_W:
+        raise _lib.TdtmError(f"image {image_h}x{image_w} exceeds the largest search grid "
+                             f"{MAX_GRID_H}x{MAX_GRID_W} (height x width)")
+    return gh, gw
+
 
 class MatchPlan:
-    """Device plan + statistics maps for one search on one device."""
+    """Device plan + statistics maps for one search on one device.
+
+    ``H`` x ``W`` is the image; the plan runs on the FFT grid ``Hg`` x ``Wg``
+    (``search_grid``), and the statistics maps have the grid's shape (the image is their
+    top-left ``H`` x ``W`` block)."""
 
     def __init__(self, image_h: int, image_w: int, template_h: int, orientation_batch_size: int,
                  num_defocus: int, num_pixel: int, device: torch.device):
@@ -45,13 +80,17 @@ class MatchPlan:
         self.lib = _lib.load()
         self.device = device
         self.H, self.W, self.h = image_h, image_w, template_h
+        self.Hg, self.Wg = search_grid(image_h, image_w)
+        self.padded = (self.Hg, self.Wg) != (image_h, image_w)
         self.B, self.D, self.C = orientation_batch_size, num_defocus, num_pixel
         handle = _lib.c_void_p()
         with torch.cuda.device(device):
-            _lib.check(self.lib.tdtm_match_plan_create(image_h, image_w, template_h, orientation_batch_size,
+            _lib.check(self.lib.tdtm_match_plan_create(self.Hg, self.Wg, template_h, orientation_batch_size,
                                                        num_defocus, num_pixel, _lib.ctypes.byref(handle)))
         self.handle = handle
-        n = (image_h, image_w)
+        if self.padded:
+            _lib.check(self.lib.tdtm_match_plan_set_image_size(handle, image_h, image_w))
+        n = (self.Hg, self.Wg)
         self.mip = torch.empty(n, dtype=torch.float32, device=device)
         self.best_index = torch.empty(n, dtype=torch.int32, device=device)
         self.sums = [torch.empty(n, dtype=torch.float32, device=device) for _ in range(4)]
@@ -88,14 +127,38 @@ class MatchPlan:
 
     def set_inputs(self, image_dft: torch.Tensor, ctf_filters: torch.Tensor, whitening: torch.Tensor) -> None:
         st = self._stream()
-        self._image = image_dft.to(self.device, torch.complex64).contiguous()
+        if tuple(image_dft.shape) != (self.H, self.W // 2 + 1):
+            raise ValueError(f"image_dft shape {tuple(image_dft.shape)} != {(self.H, self.W // 2 + 1)}")
+        image = image_dft.to(self.device, torch.complex64).contiguous()
+        if self.padded:
+            image = self._regrid(image, st)
+        self._image = image
         self._ctf = ctf_filters.to(self.device, torch.float32).contiguous()
         self._wf = whitening.to(self.device, torch.float32).contiguous()
         _lib.check(self.lib.tdtm_match_set_image(self.handle, _lib.ptr(self._image), st))
         _lib.check(self.lib.tdtm_match_set_filters(self.handle, _lib.ptr(self._ctf), _lib.ptr(self._wf), st))
 
+    def _regrid(self, image: torch.Tensor, st) -> torch.Tensor:
+        """The spectrum of the image zero-padded into the grid: any-size inverse DFT
+        into a zeroed grid-size array, then the grid's rfft2 (both on the device)."""
+        with torch.cuda.device(self.device):
+            real = torch.zeros((self.Hg, self.Wg), dtype=torch.float32, device=self.device)
+            nbytes = self.lib.tdtm_irfft2_any_workspace_bytes(self.H, self.W)
+            if nbytes < 0:
+                raise _lib.TdtmError(f"image {self.H}x{self.W}: unsupported size for re-gridding")
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            _lib.check(self.lib.tdtm_irfft2_any(_lib.ptr(image), self.H, self.W, _lib.ptr(real), self.Wg,
+                                                _lib.ptr(ws), st))
+            out = torch.empty((self.Hg, self.Wg // 2 + 1), dtype=torch.complex64, device=self.device)
+            _lib.check(self.lib.tdtm_rfft2(_lib.ptr(real), self.Hg, self.Wg, _lib.ptr(out), st))
+        return out
+
+    def image_view(self, t: torch.Tensor) -> torch.Tensor:
+        """The image's H x W block of a grid-size map."""
+        return t[:self.H, :self.W] if self.padded else t
+
     def reset_stats(self) -> None:
-        _lib.check(self.lib.tdtm_stats_init(self.H * self.W, _lib.ptr(self.mip), _lib.ptr(self.best_index),
+        _lib.check(self.lib.tdtm_stats_init(self.Hg * self.Wg, _lib.ptr(self.mip), _lib.ptr(self.best_index),
                                             *[_lib.ptr(s) for s in self.sums], self._stream()))
 
     def step(self, template_dft: torch.Tensor, euler_deg: torch.Tensor, start: int, count: int) -> None:
@@ -113,7 +176,7 @@ class MatchPlan:
         if other.device != self.device:
             tens = [t.to(self.device) for t in tens]
         _lib.check(self.lib.tdtm_stats_merge(
-            self.H * self.W, _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums],
+            self.Hg * self.Wg, _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums],
             *[_lib.ptr(t) for t in tens], self._stream()))
         self._keep = tens
 
@@ -155,15 +218,19 @@ class MatchPlan:
         return int(meta["next_orientation"])
 
     def finalize(self, euler_deg: torch.Tensor, defocus_values: torch.Tensor, total: int) -> dict:
+        """Output maps of the image (H x W): mip plus the decoded / scaled maps."""
         dev = self.device
+        maps = [self.mip, self.best_index, *self.sums]
+        if self.padded:
+            maps = [self.image_view(m).contiguous() for m in maps]
         outs = [torch.empty((self.H, self.W), dtype=torch.float32, device=dev) for _ in range(7)]
         _lib.check(self.lib.tdtm_match_finalize(
-            self.H * self.W, total, _lib.ptr(self.mip), _lib.ptr(self.best_index),
-            *[_lib.ptr(s) for s in self.sums], _lib.ptr(euler_deg), euler_deg.shape[0],
+            self.H * self.W, total, _lib.ptr(maps[0]), _lib.ptr(maps[1]),
+            *[_lib.ptr(s) for s in maps[2:]], _lib.ptr(euler_deg), euler_deg.shape[0],
             _lib.ptr(defocus_values), defocus_values.shape[0], *[_lib.ptr(o) for o in outs], self._stream()))
         keys = ["scaled_mip", "correlation_mean", "correlation_variance", "best_phi", "best_theta", "best_psi",
                 "best_defocus"]
-        return dict(zip(keys, outs))
+        return dict(zip(keys, outs), mip=maps[0])
 
 
 def _as_device_list(device) -> list[torch.device]:
@@ -253,7 +320,7 @@ def _finish(main: MatchPlan, defocus_values: torch.Tensor, O: int, D: int, C: in
         maps = main.finalize(main._eul, dvals, total)
         torch.cuda.synchronize(main.device)
     return {
-        "mip": main.mip.cpu(),
+        "mip": maps["mip"].cpu(),
         "scaled_mip": maps["scaled_mip"].cpu(),
         "best_phi": maps["best_phi"].cpu(),
         "best_theta": maps["best_theta"].cpu(),

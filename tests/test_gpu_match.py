@@ -90,6 +90,43 @@ def test_match_parity_against_oracle(case, cuda_device):
     assert got["total_projections"] == O * D * C
 
 
+NON_POW2 = [
+    # H, W, h, D, C, O, B -> search grid (next powers of two)
+    (200, 300, 32, 2, 1, 8, 4),     # 256 x 512
+    (100, 128, 64, 1, 1, 6, 3),     # 128 x 128 (one edge a power of two)
+    (640, 960, 64, 2, 1, 4, 4),     # 1024 x 1024 (four-step transforms)
+    (1000, 5000, 64, 1, 1, 4, 4),   # 1024 x 8192 (8192-sample rows)
+]
+
+
+@pytest.mark.parametrize("case", NON_POW2, ids=lambda c: "H{}W{}h{}D{}C{}O{}B{}".format(*c))
+def test_match_non_power_of_two_image(case, cuda_device):
+    """Image sizes that are not powers of two run zero-padded in the next power-of-two
+    grid: the valid region (template footprint inside the image) matches the oracle's
+    circular search at the image's own size."""
+    H, W, h, D, C, O, B = case
+    print(f"non-power-of-two case {case}", flush=True)
+    args = _inputs(H, W, h, D, C, O, seed=H + 3 * W + h)
+    got = core_match_template(*args, device=cuda_device, orientation_batch_size=B)
+    r32 = osr.core_match_template(*args, orientation_batch_size=B)
+    r64 = osr.core_match_template(*args, orientation_batch_size=B, dtype=torch.float64)
+    assert tuple(got["mip"].shape) == (H, W)
+    vy, vx = H - h + 1, W - h + 1
+    scale = float(r64["mip"][:vy, :vx].abs().max())
+    e32 = float((got["mip"][:vy, :vx].double() - r32["mip"][:vy, :vx].double()).abs().max()) / scale
+    e64 = float((got["mip"][:vy, :vx].double() - r64["mip"][:vy, :vx]).abs().max()) / scale
+    print(f"valid-region mip rel err vs fp32 {e32:.2e} vs fp64 {e64:.2e}", flush=True)
+    assert e32 < TOL and e64 < TOL
+    for k in ("correlation_mean", "correlation_variance"):
+        ref = r64[k][:vy, :vx]
+        err = float((got[k][:vy, :vx].double() - ref).abs().max() / ref.abs().max())
+        assert err < TOL, k
+    ok = (_top2_margin(args) > TOL * scale)[:vy, :vx]
+    for k in ("best_phi", "best_theta", "best_psi", "best_defocus"):
+        agree = (got[k][:vy, :vx] == r32[k][:vy, :vx]) | ~ok
+        assert bool(agree.all()), f"{k}: {int((~agree).sum())} pixels differ"
+
+
 def test_match_partial_last_batch_exact_index(cuda_device):
     """O % B != 0 with D > 1: the HIP path decodes every correlation to its true index."""
     args = _inputs(256, 256, 32, 3, 1, 10, seed=7)

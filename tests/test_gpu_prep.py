@@ -169,7 +169,7 @@ def test_prep_rejects_bad_sizes(cuda_device):
     with pytest.raises(_lib.TdtmError, match="power of two"):
         pp.rfft2_device(torch.zeros((48, 64), device=cuda_device))
     with pytest.raises(_lib.TdtmError, match="power of two"):
-        pp.rfft2_device(torch.zeros((64, 8192), device=cuda_device))
+        pp.rfft2_device(torch.zeros((64, 16384), device=cuda_device))
     with pytest.raises(_lib.TdtmError, match="power of two"):
         pp.volume_rfft_device(torch.zeros((48, 48, 48), device=cuda_device))
     with pytest.raises(_lib.TdtmError, match="unsupported image size"):

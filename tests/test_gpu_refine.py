@@ -34,7 +34,7 @@ def _particles(n, box, h, seed):
     return (torch.stack(stack), tdft, torch.tensor(np.array(poses), dtype=torch.float32), mean, std)
 
 
-@pytest.mark.parametrize("batch,box,h", [(4, 128, 32), (32, 128, 32), (8, 1024, 64)])
+@pytest.mark.parametrize("batch,box,h", [(4, 128, 32), (32, 128, 32), (8, 1024, 64), (8, 200, 64), (8, 528, 128)])
 def test_refine_parity_against_oracle(batch, box, h, cuda_device):
     n = 3
     stack, tdft, poses, mean, std = _particles(n, box, h, seed=5)

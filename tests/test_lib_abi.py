@@ -70,3 +70,30 @@ def test_prep_validation_errors_before_any_device_call():
     params = (ctypes.c_double * 6)(300.0, 2.7, 1.5, 0.0, 1.0, 0.0)  # amplitude contrast >= 1
     assert lib.tdtm_ctf_filter_stack(64, 1, 1, 1e4, 1e4, 0.0, fake, fake, params, fake, None) != 0
     assert b"CTF" in lib.tdtm_last_error()
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libtdtm.so not built")
+def test_regrid_validation_errors_before_any_device_call():
+    lib = _lib.load()
+    assert lib.tdtm_irfft2_any_workspace_bytes(4092, 5760) == 4092 * 2881 * 16
+    assert lib.tdtm_irfft2_any_workspace_bytes(5000, 64) == -1   # height above 4096
+    assert lib.tdtm_irfft2_any_workspace_bytes(100, 101) == -1   # odd width
+    fake = ctypes.c_void_p(16)  # never dereferenced: every check below fails first
+    assert lib.tdtm_irfft2_any(fake, 100, 128, fake, 64, fake, None) != 0
+    assert b"stride" in lib.tdtm_last_error()
+    assert lib.tdtm_irfft2_any(fake, 100, 8194, fake, 9000, fake, None) != 0
+    assert b"irfft2_any" in lib.tdtm_last_error()
+    h = ctypes.c_void_p()
+    assert lib.tdtm_match_plan_create(4096, 16384, 512, 8, 13, 1, ctypes.byref(h)) != 0
+    assert b"power of two" in lib.tdtm_last_error()
+
+
+def test_search_grid_rule():
+    from leopard_em_amd.backend.core_match_template import search_grid
+
+    assert search_grid(4096, 4096) == (4096, 4096)
+    assert search_grid(4092, 5760) == (4096, 8192)
+    assert search_grid(200, 300) == (256, 512)
+    assert search_grid(32, 32) == (64, 64)
+    with pytest.raises(_lib.TdtmError, match="exceeds"):
+        search_grid(5760, 4092)
