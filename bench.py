@@ -24,8 +24,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "leopard-em_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Published H100 80 GB rate of the reference on the same workload (docs/benchmarks.md:32).
-REFERENCE_H100_CORR_PER_S = 1650.8
+# Published H100 80 GB rates of the reference (docs/benchmarks.md:32, :42), by image (H, W)
+REFERENCE_H100_CORR_PER_S = {(4096, 4096): 1650.8, (4092, 5760): 897.9}
 TRAFFIC_SUMMARY = os.path.join(REPO, "profiles", "match_step_4096x4096_t512_b8_d13_traffic.md")
 
 
@@ -46,7 +46,11 @@ def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
     mg = synthetic.make_micrograph(t, seed, (H, W), n_particles, 12200.0, 11800.0, 30.0, snr=0.05,
                                    defocus_spread=3000.0, device=device)
     image = mg.image.to(device=device, dtype=torch.float32).contiguous()
-    img_dft, wf = pp.prepare_image(image, h)
+    pow2 = all(n & (n - 1) == 0 for n in (H, W))
+    # the device preparation kernels take power-of-two images; other sizes are prepared
+    # on the host (torch, as the reference does) and moved to the device
+    img_dft, wf = pp.prepare_image(image if pow2 else image.cpu(), h)
+    img_dft, wf = img_dft.to(device), wf.to(device)
     dfs = torch.arange(-1200.0, 1201.0, 200.0)
     ctf = pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]), device=device)
     return img_dft, tdft, ctf, wf.to(device), dfs, vol, image
@@ -197,6 +201,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--batch", type=int, default=8, help="orientations per step")
     ap.add_argument("--size", type=int, default=4096, help="image edge (H = W)")
+    ap.add_argument("--image", default=None,
+                    help="image HxW, e.g. 4092x5760 (K3; run in a 4096 x 8192 grid); overrides --size")
     ap.add_argument("--template", type=int, default=512)
     ap.add_argument("--cpu-orientations", type=int, default=16, help="CPU baseline sample (x 13 defocus)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -227,6 +233,8 @@ def main():
 
     dev = torch.device("cuda:0")
     H = W = args.size
+    if args.image:
+        H, W = (int(v) for v in args.image.lower().split("x"))
     h = args.template
     print(f"[bench] building {args.inputs} inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
     if args.inputs == "synthetic":
@@ -299,9 +307,10 @@ def main():
               file=sys.stderr, flush=True)
 
     traffic = read_traffic(TRAFFIC_SUMMARY) if (H, W, h, B, D) == (4096, 4096, 512, 8, 13) else None
+    ref_rate = REFERENCE_H100_CORR_PER_S.get((H, W))
 
     prep = None
-    if args.inputs == "synthetic" and not args.no_prep_timing:
+    if args.inputs == "synthetic" and not args.no_prep_timing and not plan.padded:
         prep = time_preparation(image, vol, h, dfs, dev)
         print("[bench] preparation ms: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in prep.items()),
               file=sys.stderr, flush=True)
@@ -315,7 +324,8 @@ def main():
                          f"{H}x{W} x {h}^3 in {dt:.1f} s, torch CPU float32 oracle"}
 
     line = {
-        "metric": "(orientation x defocus) correlations/s on 4096^2 image",
+        "metric": ("(orientation x defocus) correlations/s on 4096^2 image" if (H, W) == (4096, 4096)
+                   else f"(orientation x defocus) correlations/s on {H}x{W} image"),
         "value": value,
         "unit": "correlations/s",
         "n_gpus": 1,
@@ -324,16 +334,17 @@ def main():
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": value / REFERENCE_H100_CORR_PER_S,
-        "baseline": {"value": REFERENCE_H100_CORR_PER_S, "hardware": "H100 80 GB",
-                     "source": "reference docs/benchmarks.md:32"},
+        "vs_baseline": value / ref_rate if ref_rate else None,
+        "baseline": ({"value": ref_rate, "hardware": "H100 80 GB", "source": "reference docs/benchmarks.md:32-42"}
+                     if ref_rate else None),
         "dtype": "f32",
         "data": ("synthetic (seeded Gaussian-blob template and micrograph; random orientations)"
                  if args.inputs == "synthetic" else "synthetic (random spectra; random orientations)"),
         "config": {
             "workload": f"plain 2DTM search, {H}x{W} micrograph x {h}^3 template, {D} defocus, "
                         f"{B} orientations per step",
-            "image": [H, W], "template": h, "defocus": D, "orientation_batch": B,
+            "image": [H, W], "search_grid": [plan.Hg, plan.Wg], "template": h, "defocus": D,
+            "orientation_batch": B,
             "orientations": "strided slice of the 2.5/1.5 deg uniform grid",
             "full_grid_orientations": full_grid,
             "full_grid_hours_extrapolated": full_grid * D / value / 3600.0,
