@@ -46,11 +46,7 @@ def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
     mg = synthetic.make_micrograph(t, seed, (H, W), n_particles, 12200.0, 11800.0, 30.0, snr=0.05,
                                    defocus_spread=3000.0, device=device)
     image = mg.image.to(device=device, dtype=torch.float32).contiguous()
-    pow2 = all(n & (n - 1) == 0 for n in (H, W))
-    # the device preparation kernels take power-of-two images; other sizes are prepared
-    # on the host (torch, as the reference does) and moved to the device
-    img_dft, wf = pp.prepare_image(image if pow2 else image.cpu(), h)
-    img_dft, wf = img_dft.to(device), wf.to(device)
+    img_dft, wf = pp.prepare_image(image, h)
     dfs = torch.arange(-1200.0, 1201.0, 200.0)
     ctf = pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]), device=device)
     return img_dft, tdft, ctf, wf.to(device), dfs, vol, image
@@ -310,7 +306,7 @@ def main():
     ref_rate = REFERENCE_H100_CORR_PER_S.get((H, W))
 
     prep = None
-    if args.inputs == "synthetic" and not args.no_prep_timing and not plan.padded:
+    if args.inputs == "synthetic" and not args.no_prep_timing:
         prep = time_preparation(image, vol, h, dfs, dev)
         print("[bench] preparation ms: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in prep.items()),
               file=sys.stderr, flush=True)

@@ -224,9 +224,10 @@ int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins);
  *   image_dft = X * whitening(H, W) normalised to unit real-space variance
  *                                                     (utils/image_processing.py:16-75)
  *   whitening_template = whitening on the (h, h/2+1) grid  (match_template_manager.py:169-177)
- * image float32 (H, W); image_dft complex64 (H, W/2+1); whitening_template float32
- * (h, h/2+1); workspace of tdtm_prepare_image_workspace_bytes(H, W, num_freq_bins)
- * device bytes.
+ * image float32 (H, W), H <= 4096 by an even W <= 8192 (powers of two >= 32 take the
+ * FFT kernels, other sizes direct double-precision DFTs); image_dft complex64
+ * (H, W/2+1); whitening_template float32 (h, h/2+1); workspace of
+ * tdtm_prepare_image_workspace_bytes(H, W, num_freq_bins) device bytes.
  */
 int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
                        void* image_dft, float* whitening_template, void* workspace, void* stream);

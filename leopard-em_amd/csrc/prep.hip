@@ -178,8 +178,10 @@ __global__ void k_fill_ones(double* p, int n) {
   if (i < n) p[i] = 1.0;
 }
 
-// fftfreq of index i for an n-point grid (n a power of two: the quotient is exact)
-__device__ __forceinline__ double fftfreq(int i, int n) { return (double)(i < (n + 1) / 2 ? i : i - n) / (double)n; }
+// fftfreq / rfftfreq of index i for an n-point grid, as torch computes them: the signed
+// index times the rounded reciprocal 1/n (exact for powers of two)
+__device__ __forceinline__ double fftfreq(int i, int n) { return (double)(i < (n + 1) / 2 ? i : i - n) * (1.0 / n); }
+__device__ __forceinline__ double rfftfreq(int j, int n) { return (double)j * (1.0 / n); }
 
 // Radial power spectrum of an unshifted (H, W/2+1) rfft (config/correlation_filters.py:
 // 97-108 -> torch_fourier_filter whitening_filter): for every element with
@@ -197,7 +199,9 @@ __global__ __launch_bounds__(256) void k_radial_power(const float2* __restrict__
   const double rlo = fmax((b - 0.5) / scale, 0.0);
   const double rhi = fmin((b + 0.5) / scale, fm);
   const int ry = min(H / 2, (int)ceil(rhi * H) + 1);
-  const int i0 = max(-ry, -H / 2 + 1), i1 = ry;  // ky = i / H, each row once
+  // signed row frequencies i (ky = i / H): [-H/2, H/2 - 1] for even H (row H/2 is -1/2,
+  // visited here as i = H/2), [-(H-1)/2, (H-1)/2] for odd H; each row once
+  const int i0 = max(-ry, (H & 1) ? -(H / 2) : -(H / 2) + 1), i1 = ry;
   double s = 0.0, c = 0.0;
   for (int i = i0 + (int)threadIdx.x; i <= i1; i += blockDim.x) {
     const int iy = i < 0 ? i + H : i;
@@ -211,7 +215,7 @@ __global__ __launch_bounds__(256) void k_radial_power(const float2* __restrict__
     const int j0 = max(0, (int)floor(jl) - 1), j1 = min(W / 2, (int)ceil(jh) + 1);
     const float2* row = X + (long)iy * wr;
     for (int j = j0; j <= j1; ++j) {
-      const double kx = (double)j / (double)W;
+      const double kx = rfftfreq(j, W);
       const double f = sqrt(ky * ky + kx * kx);
       if (f <= fm && (int)rint(f / fm * (double)(nb - 1)) == b) {
         const float2 v = row[j];
@@ -277,7 +281,7 @@ __global__ void k_whiten_eval(const double* __restrict__ curve, int nb, double f
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)h * wr) return;
   const int iy = (int)(e / wr), jx = (int)(e % wr);
-  out[e] = whiten_at(fftfreq(iy, h), (double)jx / (double)w, curve, nb, fm);
+  out[e] = whiten_at(fftfreq(iy, h), rfftfreq(jx, w), curve, nb, fm);
 }
 
 // X *= whitening(ky, kx) in place (complex64 x float32, as utils/image_processing.py:49)
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(FILTER_BLOCK) void k_prep_filter(float2* __restrict
     const long e = base + (long)i * FILTER_BLOCK + threadIdx.x;
     if (e < n) {
       const int iy = (int)(e / wr), jx = (int)(e % wr);
-      const float wv = whiten_at(fftfreq(iy, H), (double)jx / (double)W, curve, nb, fm);
+      const float wv = whiten_at(fftfreq(iy, H), rfftfreq(jx, W), curve, nb, fm);
       float2 v = X[e];
       v = make_float2(v.x * wv, v.y * wv);
       X[e] = v;
@@ -401,6 +405,55 @@ __global__ __launch_bounds__(DFT_BLOCK) void k_idft_rows(const double2* __restri
   }
 }
 
+// Y[y][kx] = sum_x x[y][x] e^{-2 pi i kx x / W}, kx <= W/2, for every row y of a real
+// (H, W) image (any even W <= DFT_MAX_W); one block per row, the row staged in LDS.
+__global__ __launch_bounds__(DFT_BLOCK) void k_dft_rows_fwd(const float* __restrict__ img, int W,
+                                                           double2* __restrict__ Y) {
+  __shared__ double row[DFT_MAX_W];
+  const int y = blockIdx.x;
+  const int Wr = W / 2 + 1;
+  for (int j = threadIdx.x; j < W; j += DFT_BLOCK) row[j] = img[(long)y * W + j];
+  __syncthreads();
+  for (int kx = threadIdx.x; kx < Wr; kx += DFT_BLOCK) {
+    double sn, cs;
+    sincospi(-2.0 * (double)kx / (double)W, &sn, &cs);  // e^{-2 pi i kx / W}
+    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    for (int x = 0; x < W; ++x) {
+      const double v = row[x];
+      ar = fma(v, wr, ar);
+      ai = fma(v, wi, ai);
+      const double nr = fma(wr, cs, -wi * sn);
+      wi = fma(wr, sn, wi * cs);
+      wr = nr;
+    }
+    Y[(long)y * Wr + kx] = make_double2(ar, ai);
+  }
+}
+
+// X[ky][kx] = sum_y Y[y][kx] e^{-2 pi i ky y / H} for every column kx < Wr (any H <=
+// DFT_MAX_H), written as complex64; one block per column, the column staged in LDS.
+__global__ __launch_bounds__(DFT_BLOCK) void k_dft_cols_fwd(const double2* __restrict__ Y, int H, int Wr,
+                                                           float2* __restrict__ X) {
+  __shared__ double2 col[DFT_MAX_H];
+  const int kx = blockIdx.x;
+  for (int j = threadIdx.x; j < H; j += DFT_BLOCK) col[j] = Y[(long)j * Wr + kx];
+  __syncthreads();
+  for (int ky = threadIdx.x; ky < H; ky += DFT_BLOCK) {
+    double sn, cs;
+    sincospi(-2.0 * (double)ky / (double)H, &sn, &cs);
+    double wr = 1.0, wi = 0.0, ar = 0.0, ai = 0.0;
+    for (int y = 0; y < H; ++y) {
+      const double2 c = col[y];
+      ar = fma(c.x, wr, fma(-c.y, wi, ar));
+      ai = fma(c.x, wi, fma(c.y, wr, ai));
+      const double nr = fma(wr, cs, -wi * sn);
+      wi = fma(wr, sn, wi * cs);
+      wr = nr;
+    }
+    X[(long)ky * Wr + kx] = make_float2((float)ar, (float)ai);
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int M, bool SHIFT>
 int launch_rows_m(const float* x, long rows, const float2* tw, float2* out, hipStream_t st) {
@@ -464,8 +517,15 @@ int rfft2_impl(const float* image, int H, int W, float2* out, hipStream_t st) {
 
 struct PrepWorkspace {
   double *sums, *cnts, *curve, *part;
+  double2* dft;  // any-size images: the row transforms (H, W/2+1)
   int nb, npart;
 };
+
+bool prep_pow2(int H, int W) { return is_pow2(H) && is_pow2(W) && H >= 32 && W >= 32; }
+
+bool prep_size_ok(int H, int W) {
+  return H >= 2 && W >= 2 && !(W & 1) && H <= DFT_MAX_H && W <= DFT_MAX_W;
+}
 
 int num_bins(int H, int W, int requested) { return requested > 0 ? requested : (H < W ? H : W) / 2 + 1; }
 
@@ -483,6 +543,8 @@ PrepWorkspace carve(void* ws, int H, int W, int requested_bins) {
   w.cnts = p + w.nb;
   w.curve = p + 2 * w.nb;
   w.part = p + 3 * w.nb;
+  const long used = 3L * w.nb + w.npart;
+  w.dft = reinterpret_cast<double2*>(p + ((used + 1) & ~1L));  // 16-byte aligned
   return w;
 }
 
@@ -537,16 +599,19 @@ int tdtm_irfft2_any(const void* spectrum, int H, int W, float* image, int64_t ld
 }
 
 int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins) {
-  if (!is_pow2(H) || !is_pow2(W) || H < 32 || W < 32 || H > 4096 || W > 4096) return -1;
+  if (!prep_size_ok(H, W)) return -1;
   if (num_freq_bins == 1 || num_freq_bins > 65536) return -1;
-  return (int64_t)(3 * num_bins(H, W, num_freq_bins) + filter_blocks(H, W)) * (int64_t)sizeof(double);
+  int64_t doubles = 3L * num_bins(H, W, num_freq_bins) + filter_blocks(H, W);
+  if (!prep_pow2(H, W)) doubles = ((doubles + 1) & ~1L) + 2L * H * (W / 2 + 1);
+  return doubles * (int64_t)sizeof(double);
 }
 
 int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
                        void* image_dft, float* whitening_template, void* workspace, void* stream) {
   if (!image || !image_dft || !whitening_template || !workspace) return fail("null argument");
-  if (!is_pow2(H) || !is_pow2(W) || H < 32 || W < 32 || H > 4096 || W > 4096)
-    return fail("image size must be a power of two in [32, 4096], got " + std::to_string(H) + "x" + std::to_string(W));
+  if (!prep_size_ok(H, W))
+    return fail("image size must be H <= 4096 by an even W <= 8192, got " + std::to_string(H) + "x" +
+                std::to_string(W));
   if (h < 2 || (h & 1)) return fail("template size must be even and >= 2");
   if (num_freq_bins == 1 || num_freq_bins > 65536) return fail("num_freq_bins must be 0 (default) or in [2, 65536]");
   if (!(max_freq > 0.0)) return fail("max_freq must be positive");
@@ -554,7 +619,14 @@ int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, i
   float2* X = static_cast<float2*>(image_dft);
   const PrepWorkspace w = carve(workspace, H, W, num_freq_bins);
   const double fm = max_freq;
-  if (rfft2_impl(image, H, W, X, st)) return 1;
+  if (prep_pow2(H, W)) {
+    if (rfft2_impl(image, H, W, X, st)) return 1;
+  } else {  // direct double-precision DFTs for sizes the FFT kernels do not take
+    k_dft_rows_fwd<<<H, DFT_BLOCK, 0, st>>>(image, W, w.dft);
+    TDTM_LAUNCHED();
+    k_dft_cols_fwd<<<W / 2 + 1, DFT_BLOCK, 0, st>>>(w.dft, H, W / 2 + 1, X);
+    TDTM_LAUNCHED();
+  }
   k_zero_dc<<<1, 1, 0, st>>>(X);
   TDTM_LAUNCHED();
   if (whitening) {

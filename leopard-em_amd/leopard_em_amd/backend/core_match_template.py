@@ -131,14 +131,14 @@ class MatchPlan:
             raise ValueError(f"image_dft shape {tuple(image_dft.shape)} != {(self.H, self.W // 2 + 1)}")
         image = image_dft.to(self.device, torch.complex64).contiguous()
         if self.padded:
-            image = self._regrid(image, st)
+            image = self.regrid(image, st)
         self._image = image
         self._ctf = ctf_filters.to(self.device, torch.float32).contiguous()
         self._wf = whitening.to(self.device, torch.float32).contiguous()
         _lib.check(self.lib.tdtm_match_set_image(self.handle, _lib.ptr(self._image), st))
         _lib.check(self.lib.tdtm_match_set_filters(self.handle, _lib.ptr(self._ctf), _lib.ptr(self._wf), st))
 
-    def _regrid(self, image: torch.Tensor, st) -> torch.Tensor:
+    def regrid(self, image: torch.Tensor, st) -> torch.Tensor:
         """The spectrum of the image zero-padded into the grid: any-size inverse DFT
         into a zeroed grid-size array, then the grid's rfft2 (both on the device)."""
         with torch.cuda.device(self.device):

@@ -82,7 +82,7 @@ def core_refine_template(
         for i in range(n):
             xp = particle_stack_dft[i].to(dev, torch.complex64).contiguous()
             if plan.padded:  # crop sizes that are not powers of two: spectrum of the padded crop
-                xp = plan._regrid(xp, st)
+                xp = plan.regrid(xp, st)
             pose = euler_angles[i].to(dev, torch.float32).contiguous()
             filt = projective_filters[i].to(dev, torch.float32).contiguous()
             mu = corr_mean[i].to(dev, torch.float32).contiguous()

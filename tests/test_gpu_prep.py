@@ -82,7 +82,8 @@ def test_volume_rfft_512_central_plane(cuda_device):
     assert abs(float(energy) / want - 1.0) <= 1e-5
 
 
-@pytest.mark.parametrize("shape,h", [((64, 64), 32), ((256, 512), 64), ((1024, 1024), 128), ((4096, 4096), 512)])
+@pytest.mark.parametrize("shape,h", [((64, 64), 32), ((256, 512), 64), ((1024, 1024), 128), ((4096, 4096), 512),
+                                     ((200, 300), 64), ((201, 300), 32), ((1000, 1500), 128), ((4092, 5760), 512)])
 def test_prepare_image_matches_oracle(cuda_device, shape, h):
     print(f"prepare_image {shape} h {h}", flush=True)
     t = synthetic.make_blob_template(3, h, 80, 0.35 * h)
@@ -173,4 +174,6 @@ def test_prep_rejects_bad_sizes(cuda_device):
     with pytest.raises(_lib.TdtmError, match="power of two"):
         pp.volume_rfft_device(torch.zeros((48, 48, 48), device=cuda_device))
     with pytest.raises(_lib.TdtmError, match="unsupported image size"):
-        pp.prepare_image_device(torch.zeros((100, 128), device=cuda_device), 32)
+        pp.prepare_image_device(torch.zeros((100, 127), device=cuda_device), 32)
+    with pytest.raises(_lib.TdtmError, match="unsupported image size"):
+        pp.prepare_image_device(torch.zeros((5000, 128), device=cuda_device), 32)

@@ -50,8 +50,8 @@ def test_missing_library_fails_loudly(tmp_path):
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="libtdtm.so not built")
 def test_prep_validation_errors_before_any_device_call():
     lib = _lib.load()
-    assert lib.tdtm_prepare_image_workspace_bytes(100, 128, 0) == -1
-    assert lib.tdtm_prepare_image_workspace_bytes(4096, 8192, 0) == -1
+    assert lib.tdtm_prepare_image_workspace_bytes(100, 127, 0) == -1
+    assert lib.tdtm_prepare_image_workspace_bytes(4096, 16384, 0) == -1
     assert lib.tdtm_prepare_image_workspace_bytes(64, 64, 1) == -1
     # (3 x 33 bins + 1 filter block) doubles for 64 x 64
     assert lib.tdtm_prepare_image_workspace_bytes(64, 64, 0) == (3 * 33 + 1) * 8
