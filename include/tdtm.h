@@ -70,8 +70,8 @@ int tdtm_match_plan_set_image_size(tdtm_match_plan_t plan, int image_h, int imag
  * iteration sharing twiddles and barriers; default 0), "colpass_chunks" (correlation chunks
  * per column group, 0 = automatic), "colpass_twiddles" (0: global table, 1: LDS copy
  * for the four-step column transforms where it fits, 2: each lane's twiddles built once
- * in registers for all its correlations, default; two-column blocks with one correlation
- * per iteration, otherwise the LDS copy), "colpass_store" (0: direct
+ * in registers for all its correlations, default; one correlation per iteration,
+ * otherwise the LDS copy), "colpass_store" (0: direct
  * 8-byte stores, default; 1: rows of both columns through LDS, 16-byte full-tile stores), "rowpass_staging" (0: synchronous de-interleaved,
  * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat,
  * 3: synchronous de-interleaved with batched loads; default 3), "rowpass_twiddles" (0: global

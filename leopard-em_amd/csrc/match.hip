@@ -1406,7 +1406,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
-    constexpr bool regs_ = FOUR && CWB_ == CW && NT_ == 1;                                \
+    constexpr bool regs_ = FOUR && NT_ == 1;                                              \
     if (rtw && regs_) {                                                                   \
       if (full) TDTM_COLPASS_T(CWB_, NT_, (regs_ ? 2 : 0), true);                         \
       else TDTM_COLPASS_T(CWB_, NT_, (regs_ ? 2 : 0), false);                             \
