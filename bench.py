@@ -52,7 +52,7 @@ def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
     return img_dft, tdft, ctf, wf.to(device), dfs, vol, image
 
 
-def time_preparation(image, vol, h, dfs, device, reps=5):
+def time_preparation(image, vol, h, dfs, device, reps=5, cpu_baseline=False):
     """Device time of the once-per-search input preparation (image spectrum + whitening,
     template 3D rfft, CTF stack) and its rate against the minimum HBM traffic: the
     input read once and the output written once."""
@@ -87,6 +87,21 @@ def time_preparation(image, vol, h, dfs, device, reps=5):
         ms = timed(fn)
         gbs = nbytes / (ms / 1e3) / 1e9
         out[name] = {"ms": ms, "min_bytes": nbytes, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    if cpu_baseline:
+        # the CPU oracle's restatement of the reference's host preparation, all host threads
+        sys.path.insert(0, REPO)
+        from oracle import filters as of
+
+        torch.set_num_threads(os.cpu_count() or 1)
+        img_c, vol_c = image.cpu(), vol.cpu()
+        t0 = time.perf_counter()
+        of.prepare_image_dft(img_c, (h, h // 2 + 1))
+        t1 = time.perf_counter()
+        of.volume_to_rfft_fourier_slice(vol_c)
+        t2 = time.perf_counter()
+        out["prepare_image"]["cpu_ms"] = (t1 - t0) * 1e3
+        out["volume_rfft"]["cpu_ms"] = (t2 - t1) * 1e3
+        out["cpu_cores"] = os.cpu_count() or 1
     return out
 
 
@@ -163,6 +178,26 @@ def bench_refine(args) -> dict:
     sec = e0.elapsed_time(e1) / 1e3
     ncorr = n * offs.shape[0] * dfo.numel() * pxo.numel()
     print(f"[bench] refine {n} particles: {sec:.2f} s, {ncorr / sec:.0f} corr/s", file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the float32 CPU oracle (reference algorithm) on one particle with a 5^3 offset grid
+        sys.path.insert(0, REPO)
+        from oracle import search as osr
+
+        cores = os.cpu_count() or 1
+        torch.set_num_threads(cores)
+        g3 = torch.tensor([-2.0, -1.0, 0.0, 1.0, 2.0])
+        offs3 = torch.stack(torch.meshgrid(g3, g3, g3, indexing="ij"), -1).reshape(-1, 3)
+        cargs = (x[:1].cpu(), tdft.cpu(), eul[:1].cpu(), offs3, dfo, du[:1], dv[:1], da[:1], pxo, mean[:1].cpu(),
+                 std[:1].cpu(), kw, filt[:1].cpu())
+        t0 = time.perf_counter()
+        osr.core_refine_template(*cargs, batch_size=32)
+        dt = time.perf_counter() - t0
+        nc = offs3.shape[0] * dfo.numel() * pxo.numel()
+        cpu = {"value": nc / dt, "unit": "correlations/s", "cores": cores, "kind": "port",
+               "sample": f"1 particle x {offs3.shape[0]} offsets x {dfo.numel()} defocus x {pxo.numel()} pixel sizes "
+                         f"= {nc} correlations in {dt:.1f} s, torch CPU float32 oracle"}
+        print(f"[bench] CPU baseline: {nc / dt:.1f} corr/s", file=sys.stderr, flush=True)
     return {
         "metric": "refinement correlations/s (512^2 crops x 256^3 template)",
         "value": ncorr / sec,
@@ -175,6 +210,7 @@ def bench_refine(args) -> dict:
                                f"{offs.shape[0]} orientation offsets x {dfo.numel()} defocus x {pxo.numel()} "
                                "pixel sizes, batches of 32 offsets",
                    "particles_per_s": n / sec},
+        "cpu_baseline": cpu,
     }
 
 
@@ -307,8 +343,9 @@ def main():
 
     prep = None
     if args.inputs == "synthetic" and not args.no_prep_timing:
-        prep = time_preparation(image, vol, h, dfs, dev)
-        print("[bench] preparation ms: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in prep.items()),
+        prep = time_preparation(image, vol, h, dfs, dev, cpu_baseline=not args.no_cpu_baseline)
+        print("[bench] preparation ms: " + ", ".join(f"{k} {v['ms']:.3f}" for k, v in prep.items()
+                                                     if isinstance(v, dict)),
               file=sys.stderr, flush=True)
 
     cpu = None
