@@ -123,8 +123,8 @@ def core_match_template(image_dft, template_dft, ctf_filters, whitening_filter_t
     The orientation loop follows ``:492-554`` with one device, so the work queue is
     the contiguous range [0, O).  Global indices use the (C, D, B) flattening of
     ``:458-466``/``:513-520``.  The final partial batch uses its true size (the
-    reference's arange(i, i+B) over-runs it, quirk noted in DESIGN.md); the two agree
-    whenever O % B == 0 or C*D == 1.
+    reference's arange(i, i+B) at ``:513-518`` over-runs it when the last batch is
+    short); the two agree whenever O % B == 0 or C*D == 1.
     """
     cdt = torch.complex128 if dtype == torch.float64 else torch.complex64
     image_dft = image_dft.to(cdt)
