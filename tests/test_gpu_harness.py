@@ -54,8 +54,11 @@ def test_resume_is_bit_exact(tmp_path, cuda_device):
         core_match_template_resumable(*other, device=cuda_device, checkpoint_path=ck, orientation_batch_size=4)
 
 
-def test_manager_yaml_mrc_end_to_end(tmp_path, cuda_device):
-    H, W, h = 256, 256, 32
+@pytest.mark.parametrize("H,W", [(256, 256), (250, 300)])
+def test_manager_yaml_mrc_end_to_end(tmp_path, cuda_device, H, W):
+    """YAML + MRC in, exported maps and peak table out; (250, 300) runs the device
+    preparation's direct DFTs and the search in a zero-padded 256 x 512 grid."""
+    h = 32
     t = synthetic.make_blob_template(21, h, 120, 11.0)
     mg = synthetic.make_micrograph(t, 21, (H, W), 3, 12200.0, 11800.0, 30.0, snr=0.5)
     mrc.write_mrc_from_tensor(mg.image.float(), tmp_path / "micrograph.mrc", voxel_size=1.0)
