@@ -61,6 +61,12 @@ CASES = [
     (1024, 2048, 64, 2, 1, 4, 4),
     (2048, 4096, 64, 2, 1, 4, 4),
     (4096, 1024, 128, 1, 1, 4, 2),
+    # edge cases: template as large as the image, one orientation per batch with three
+    # pixel sizes, a 512^3 template, many defocus planes
+    (64, 64, 64, 2, 1, 4, 2),
+    (128, 64, 32, 1, 3, 3, 1),
+    (512, 512, 512, 1, 1, 4, 2),
+    (256, 256, 32, 24, 1, 4, 4),
 ]
 
 
