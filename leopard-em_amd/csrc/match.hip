@@ -141,32 +141,47 @@ __global__ void k_slice(const float2* __restrict__ vol, int n, const float* __re
 }
 
 // Inverse FFT along y of (slice * filter) for CP columns per block.
+// Block (column group, slice b, filter chunk): the slice columns are loaded once and
+// go through the filters f0 .. f1-1 of the chunk (correlation k = b + nb f, filter f =
+// ci nd + d), the next filter's values loaded while the current transform runs.
 template <int HS, int CP>
 __global__ __launch_bounds__(CP* HS / V) void k_proj_cols(const float2* __restrict__ S,
-                                                          const float* __restrict__ filt, int nb, int nd,
-                                                          const float2* __restrict__ tw, float2* __restrict__ Tb) {
+                                                          const float* __restrict__ filt, int nb, int nf,
+                                                          int fchunk, const float2* __restrict__ tw,
+                                                          float2* __restrict__ Tb) {
   constexpr int T = HS / V;
   constexpr int WR = HS / 2 + 1;
   constexpr int LF = LdsSize<HS>::floats;
   __shared__ float lds[CP * 2 * LF];
   const int c = threadIdx.x % CP, t = threadIdx.x / CP;
   const int kx = blockIdx.x * CP + c;
-  const int k = blockIdx.y;
-  const int b = k % nb, d = (k / nb) % nd, ci = k / (nb * nd);
+  const int b = blockIdx.y;
+  const int f0 = blockIdx.z * fchunk;
+  const int f1 = min(nf, f0 + fchunk);
   const bool ok = kx < WR;
   const float2* s = S + (long)b * HS * WR;
-  const float* f = filt + ((long)ci * nd + d) * HS * WR;
-  float2 v[V];
+  float2 sv[V];
+  float fv[V];
 #pragma unroll
   for (int i = 0; i < V; ++i) {
-    const int ky = t + i * T;
-    v[i] = ok ? c_scale(s[(long)ky * WR + kx], f[(long)ky * WR + kx]) : make_float2(0.f, 0.f);
+    const long e = (long)(t + i * T) * WR + kx;
+    sv[i] = ok ? s[e] : make_float2(0.f, 0.f);
+    fv[i] = ok ? filt[(long)f0 * HS * WR + e] : 0.f;
   }
-  fft_wg<HS, V, true>(v, lds + c * 2 * LF, lds + c * 2 * LF + LF, t, tw);
-  if (ok) {
-    float2* o = Tb + (long)k * HS * WR;
+  for (int f = f0; f < f1; ++f) {
+    float2 v[V];
 #pragma unroll
-    for (int i = 0; i < V; ++i) o[(long)(t + i * T) * WR + kx] = v[i];
+    for (int i = 0; i < V; ++i) v[i] = c_scale(sv[i], fv[i]);
+    if (f + 1 < f1) {
+#pragma unroll
+      for (int i = 0; i < V; ++i) fv[i] = ok ? filt[(long)(f + 1) * HS * WR + (long)(t + i * T) * WR + kx] : 0.f;
+    }
+    fft_wg<HS, V, true>(v, lds + c * 2 * LF, lds + c * 2 * LF + LF, t, tw);
+    if (ok) {
+      float2* o = Tb + (long)(b + (long)nb * f) * HS * WR;
+#pragma unroll
+      for (int i = 0; i < V; ++i) o[(long)(t + i * T) * WR + kx] = v[i];
+    }
   }
 }
 
@@ -1341,8 +1356,13 @@ template <int HS>
 static void launch_proj(tdtm_match_plan* p, hipStream_t st) {
   constexpr int CP = (HS / V) >= 64 ? 4 : 8;
   constexpr int WR = HS / 2 + 1;
-  dim3 gc((WR + CP - 1) / CP, p->K);
-  k_proj_cols<HS, CP><<<gc, CP * (HS / V), 0, st>>>(p->S, p->filt, p->K / (p->D * p->C), p->D, p->tw, p->Tb);
+  const int nf = p->D * p->C, nb = p->K / nf;
+  const int gx = (WR + CP - 1) / CP;
+  // filter chunks: about 8 blocks per CU, each running several filters of one slice
+  int nchunk = (int)std::min<long>(nf, std::max<long>(1, (8L * p->num_cu + (long)gx * nb - 1) / ((long)gx * nb)));
+  const int fchunk = (nf + nchunk - 1) / nchunk;
+  nchunk = (nf + fchunk - 1) / fchunk;
+  k_proj_cols<HS, CP><<<dim3(gx, nb, nchunk), CP * (HS / V), 0, st>>>(p->S, p->filt, nb, nf, fchunk, p->tw, p->Tb);
   constexpr int TRW = (HS / 2) / V;
   constexpr int RPB0 = 256 / TRW;
   constexpr int RPB = RPB0 > HS ? HS : RPB0;
