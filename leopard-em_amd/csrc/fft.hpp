@@ -205,6 +205,10 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Orders the LDS accesses of one wave: they execute in order, so this only stops the
+// compiler from moving them across an exchange done by the threads of a single wave
+__device__ __forceinline__ void wave_lds_order() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
 template <int N>
 struct LdsSize {
   static constexpr int floats = N + N / 16 + 1;  // per component (index N is valid)
@@ -237,7 +241,9 @@ __device__ __forceinline__ void dft_sparse(float2 (&a)[R], TWT tw) {
 }
 
 // One Stockham pass of a size-N transform (span NS -> NS*R).  v holds x[t + i*T].
-template <int N, int V, int NS, bool INV, int NZJ, class TWT>
+// WAVE: all T threads of the transform lie in one wave, whose in-order LDS accesses need
+// no workgroup barrier.
+template <int N, int V, int NS, bool INV, int NZJ, bool WAVE = false, class TWT>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict__ lre, float* __restrict__ lim,
                                               int t, TWT tw) {
   constexpr int T = N / V;
@@ -277,23 +283,23 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[V], float* __restrict_
     }
   }
   if constexpr (!LAST) {
-    lds_barrier();
+    if constexpr (WAVE) wave_lds_order(); else lds_barrier();
 #pragma unroll
     for (int i = 0; i < V; ++i) {
       const int p = lds_pad(t + i * T);
       v[i] = make_float2(lre[p], lim[p]);
     }
-    lds_barrier();
+    if constexpr (WAVE) wave_lds_order(); else lds_barrier();
   }
 }
 
 // Full transform; NZJ describes the zero padding of the input (first pass only).
-template <int N, int V, bool INV, int NZJ = 16, int NS = 1, class TWT>
+template <int N, int V, bool INV, int NZJ = 16, int NS = 1, bool WAVE = false, class TWT>
 __device__ __forceinline__ void fft_wg(float2 (&v)[V], float* __restrict__ lre, float* __restrict__ lim, int t,
                                        TWT tw) {
   constexpr int R = (N / NS < V) ? N / NS : V;
-  stockham_pass<N, V, NS, INV, (NS == 1 ? NZJ : 16)>(v, lre, lim, t, tw);
-  if constexpr (NS * R < N) fft_wg<N, V, INV, NZJ, NS * R>(v, lre, lim, t, tw);
+  stockham_pass<N, V, NS, INV, (NS == 1 ? NZJ : 16), WAVE>(v, lre, lim, t, tw);
+  if constexpr (NS * R < N) fft_wg<N, V, INV, NZJ, NS * R, WAVE>(v, lre, lim, t, tw);
 }
 
 }  // namespace tdtm

@@ -51,11 +51,6 @@ __device__ __forceinline__ int fs_index(int t, int s) {
   return k1 + 8 * (m1 + NW * r + 8 * NW * s);
 }
 
-__device__ __forceinline__ void wave_lds_order() {
-  // LDS accesses of one wave execute in order; this only stops the compiler from
-  // moving them across the exchange
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-}
 
 template <int R, bool INV, class TWT>
 __device__ __forceinline__ void twiddle_mul(float2 (&a)[R], TWT tw, int k, int step) {

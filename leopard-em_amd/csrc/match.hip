@@ -201,8 +201,10 @@ __global__ __launch_bounds__(RPB*(HS / 2) / V) void k_proj_rows(const float2* __
   const int y = blockIdx.x * RPB + rl;
   const int k = blockIdx.y;
   const float2* row = Tb + ((long)k * HS + y) * WR;
+  static_assert(T <= 64 && 64 % T == 0, "a row's threads lie in one wave");
+  // every exchange below stays inside the row's threads, i.e. inside one wave
   for (int e = t; e < WR; e += T) stg[rl * WR + e] = row[e];
-  __syncthreads();
+  wave_lds_order();
   float2 v[V];
   const float2* g = stg + rl * WR;
 #pragma unroll
@@ -220,8 +222,8 @@ __global__ __launch_bounds__(RPB*(HS / 2) / V) void k_proj_rows(const float2* __
     const float2 od = c_mul(w, ddif);
     v[i] = make_float2(dsum.x - od.y, dsum.y + od.x);  // dsum + i*od
   }
-  __syncthreads();
-  fft_wg<M, V, true>(v, lds + rl * 2 * LF, lds + rl * 2 * LF + LF, t, tw);
+  wave_lds_order();
+  fft_wg<M, V, true, 16, 1, true>(v, lds + rl * 2 * LF, lds + rl * 2 * LF + LF, t, tw);
   const float scale = 1.0f / ((float)HS * (float)HS);
   float s1 = 0.f, s2 = 0.f;
   float* prow = P + ((long)k * HS + y) * HS;
