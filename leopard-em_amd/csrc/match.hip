@@ -474,7 +474,8 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
 template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL, bool TS = false>
 __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int Gs, int K, int kchunk, int ystore,
-                                                         const float2* __restrict__ tw_global, float2* __restrict__ Gb) {
+                                                         const float2* __restrict__ tw_global, float2* __restrict__ Gb,
+                                                         int persist) {
   constexpr int T = HH / V;
   constexpr bool FOUR = UseFour<HH>::value;
   constexpr int HF = FOUR ? HH : 512;
@@ -491,7 +492,12 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
   // robin) and complete the lines in its L2; the grid is padded to whole groups and the
   // blocks past the last tile column exit
-  int g, c = 0;
+  // Persistent launch (persist > 0; two-column blocks, one correlation per iteration):
+  // one block per resident slot; the blocks b and b + 8 (one XCD) form a pair that takes
+  // tile columns 2m and 2m + 1 of the same work items (m, k), and pair j takes the items
+  // [j persist, (j + 1) persist) of the m-major list of ceil(G/2) x K items -- every pair
+  // gets the same number of correlations, the image column is reloaded where m changes.
+  int g = 0, c = 0;
   {
     const int b = blockIdx.x, q = b >> 3;
     if constexpr (CWB == CW) {
@@ -500,7 +506,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       c = q & 1;
       g = 2 * ((q >> 2) * 8 + (b & 7)) + ((q >> 1) & 1);
     }
-    if (g >= G) return;
+    if (persist == 0 && g >= G) return;
   }
   float2* ltw = reinterpret_cast<float2*>(lds + CWB * NT * 2 * LF);
   if constexpr (TWM == 1) {
@@ -519,19 +525,61 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       return tw_global;
   }();
   if constexpr (CWB == CW) c = cl;
-  const int k0 = blockIdx.y * kchunk;
-  const int k1 = min(K, k0 + kchunk);
-  float2 xi[V];
-#pragma unroll
-  for (int i = 0; i < V; ++i) {
-    int ky = t + i * T;
-    if constexpr (FOUR) ky = fs_index<HF>(t, i);
-    xi[i] = Xt[((long)g * HH + ky) * CW + c];
-  }
-  const bool dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
   float* base = lds + cl * NT * 2 * LF;
   float2* lds2 = reinterpret_cast<float2*>(lds);
   static_assert(!TS || (CWB == CW && NT == 1 && FOUR && 2 * HH <= CW * LF), "transposed stores: layout");
+  float2 xi[V];
+  // image column g (permuted order on the four-step path)
+  auto load_image = [&](int gg) {
+#pragma unroll
+    for (int i = 0; i < V; ++i) {
+      int ky = t + i * T;
+      if constexpr (FOUR) ky = fs_index<HF>(t, i);
+      xi[i] = Xt[((long)gg * HH + ky) * CW + c];
+    }
+  };
+  if constexpr (FOUR && NT == 1 && CWB == CW) {
+    // correlations k0 .. k1-1 of tile column gg, the first iteration peeled so that the
+    // loop is always entered with the prefetched inputs followed by one iteration's
+    // stores in the memory queue (the wait for the inputs then leaves the stores in flight)
+    auto segment = [&](int gg, int k0, int k1) {
+      load_image(gg);
+      const bool dc = (gg == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0
+      float* const re1[1] = {base};
+      float* const im1[1] = {base + LF};
+      float2 in1[1][NL];
+      const int klast = k1 - 1;
+      int k = k0;
+      colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, gg, c, t, k, klast);
+      colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb,
+                                            lds2);
+      for (++k; k < k1; ++k)
+        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1, tw,
+                                              Gb, lds2);
+    };
+    if (persist > 0) {
+      const int b = blockIdx.x, q = b >> 3;
+      const int half = q & 1, pair = (q >> 1) * 8 + (b & 7);
+      const long total = (long)((G + 1) / 2) * K;
+      long item = (long)pair * persist;
+      const long iend = min(total, item + persist);
+      while (item < iend) {
+        const int m = (int)(item / K), k0 = (int)(item % K);
+        const int k1 = (int)min((long)K, k0 + (iend - item));
+        if (2 * m + half < G) segment(2 * m + half, k0, k1);  // uniform over the block
+        item += k1 - k0;
+      }
+    } else {
+      const int k0 = blockIdx.y * kchunk;
+      const int k1 = min(K, k0 + kchunk);
+      if (k0 < k1) segment(g, k0, k1);
+    }
+    return;
+  }
+  const int k0 = blockIdx.y * kchunk;
+  const int k1 = min(K, k0 + kchunk);
+  load_image(g);
+  const bool dc = (g == 0 && c == 0 && t == 0);  // ky = 0 sits in slot 0 of thread 0 in both orders
   if constexpr (FOUR) {
     float* const re1[1] = {base};
     float* const im1[1] = {base + LF};
@@ -1303,7 +1351,8 @@ struct tdtm_match_plan {
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
   int opt_colpass_chunks = 0;   // k-chunks per column group (0 = automatic)
   int opt_colpass_twiddles = 2; // four-step column transforms: 0 global table, 1 LDS copy, 2 registers
-  int opt_colpass_store = 0;    // two-column blocks: 0 direct 8-byte stores, 1 whole rows via LDS
+  int opt_colpass_store = 1;    // two-column blocks: 0 direct 8-byte stores, 1 whole rows via LDS
+  int opt_colpass_persistent = 1;  // two-column blocks, one correlation per iteration: resident-sized grid
   int opt_rowpass_staging = 3;  // 0: synchronous de-interleaved, 1: async raw, 2: async raw + reformat,
                                 // 3: synchronous de-interleaved with batched loads
   int opt_rowpass_twiddles = 1; // 0: twiddles from the global table, 1: LDS copy (staging 0 / 3 only)
@@ -1414,7 +1463,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   const bool full = ystore == HH;
   const bool ltw = FOUR && p->opt_colpass_twiddles == 1;
   const bool rtw = FOUR && p->opt_colpass_twiddles == 2;
-  using Kern = void (*)(const float2*, const float2*, int, int, int, int, int, int, const float2*, float2*);
+  using Kern = void (*)(const float2*, const float2*, int, int, int, int, int, int, const float2*, float2*, int);
   Kern kern = nullptr;
 #define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                       \
   do {                                                                                             \
@@ -1451,6 +1500,21 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
 #undef TDTM_COLPASS
 #undef TDTM_COLPASS_T
   const int threads = cols * (HH / V);
+  if (FOUR && cols == CW && nt == 1 && p->opt_colpass_persistent) {
+    // persistent schedule: as many blocks as are resident at once (a multiple of 16,
+    // pairs b, b + 8), each pair taking an equal share of the ceil(G/2) x K items
+    int per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1)
+      per_cu = 1;
+    long nblk = ((long)per_cu * p->num_cu) / 16 * 16;
+    if (nblk < 16) nblk = 16;
+    const long total = (long)((p->G + 1) / 2) * K;
+    const long npairs = nblk / 2;
+    const int persist = (int)((total + npairs - 1) / npairs);
+    hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(threads), 0, st, (const float2*)p->Rb,
+                       (const float2*)p->Xt, p->h, p->G, p->Gs, K, K, ystore, (const float2*)p->tw, p->Gb, persist);
+    return;
+  }
   // correlation chunks per column group: the split that minimises (rounds of resident
   // blocks) x (correlations per block + 2 for the block's set-up: image column, twiddle
   // table, first load), i.e. the last round of blocks is as full as possible without
@@ -1480,7 +1544,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   if (nt == 2 && (kchunk & 1) && kchunk < K) ++kchunk;  // keep pairs inside a chunk
   nchunk = (K + kchunk - 1) / kchunk;
   hipLaunchKernelGGL(kern, dim3(nblk_x, nchunk), dim3(threads), 0, st, (const float2*)p->Rb, (const float2*)p->Xt,
-                     p->h, p->G, p->Gs, K, kchunk, ystore, (const float2*)p->tw, p->Gb);
+                     p->h, p->G, p->Gs, K, kchunk, ystore, (const float2*)p->tw, p->Gb, 0);
 }
 
 template <int HH>
@@ -1687,6 +1751,8 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
   } else if (k == "colpass_twiddles") {
     if (value < 0 || value > 2) return fail("colpass_twiddles must be 0, 1 or 2");
     p->opt_colpass_twiddles = value;
+  } else if (k == "colpass_persistent") {
+    p->opt_colpass_persistent = value != 0;
   } else if (k == "colpass_chunks") {
     if (value < 0) return fail("colpass_chunks must be >= 0");
     p->opt_colpass_chunks = value;

@@ -34,7 +34,9 @@ CANDIDATES: list[dict[str, int]] = [
     {"rowpass_twiddles": 0},
     {"rowpass_rows": 2},
     {"rowpass_rows": 4},
-    {"colpass_store": 1},
+    {"colpass_store": 0},
+    {"colpass_persistent": 0},
+    {"colpass_store": 0, "colpass_persistent": 0},
 ]
 
 _OPTION_DEFAULTS = {
@@ -42,7 +44,8 @@ _OPTION_DEFAULTS = {
     "colpass_pair": 0,
     "colpass_chunks": 0,
     "colpass_twiddles": 2,
-    "colpass_store": 0,
+    "colpass_store": 1,
+    "colpass_persistent": 1,
     "rowpass_twiddles": 1,
     "rowpass_staging": 3,
     "rowpass_rows": 1,
@@ -59,7 +62,8 @@ def apply_options(plan, options: dict[str, int]) -> None:
     twiddles in LDS are switched off for the staging modes that leave no room)."""
     plan.set_option("rowpass_twiddles", 0)
     plan.set_option("rowpass_staging", _OPTION_DEFAULTS["rowpass_staging"])
-    for k in ("colpass_columns", "colpass_pair", "colpass_chunks", "colpass_twiddles", "colpass_store", "rowpass_rows"):
+    for k in ("colpass_columns", "colpass_pair", "colpass_chunks", "colpass_twiddles", "colpass_store",
+              "colpass_persistent", "rowpass_rows"):
         plan.set_option(k, _OPTION_DEFAULTS[k])
     want = dict(options)
     tw = want.pop("rowpass_twiddles", None)

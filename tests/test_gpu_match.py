@@ -202,6 +202,8 @@ def test_match_peak_list_identical(cuda_device):
     {"colpass_store": 1}, {"colpass_store": 1, "colpass_twiddles": 0},
     {"colpass_twiddles": 2}, {"colpass_twiddles": 2, "colpass_store": 1}, {"colpass_twiddles": 2, "colpass_pair": 1},
     {"colpass_twiddles": 2, "colpass_columns": 1}, {"colpass_twiddles": 1}, {"colpass_twiddles": 1, "colpass_store": 1},
+    {"colpass_persistent": 0}, {"colpass_persistent": 0, "colpass_store": 0}, {"colpass_store": 0},
+    {"colpass_persistent": 0, "colpass_twiddles": 0}, {"colpass_twiddles": 0, "colpass_store": 0},
     {"phase_timing": 1},
 ], ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
 def test_launch_options_do_not_change_results(opts, cuda_device):
@@ -223,8 +225,8 @@ def test_launch_options_do_not_change_results(opts, cuda_device):
         assert torch.equal(out[0][k], out[1][k]), k
 
 
-@pytest.mark.parametrize("opts", [{"colpass_twiddles": 2}, {"colpass_twiddles": 1}, {"colpass_store": 1}],
-                         ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()))
+@pytest.mark.parametrize("opts", [{}, {"colpass_twiddles": 1}, {"colpass_store": 0}, {"colpass_persistent": 0}],
+                         ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()) or "defaults")
 def test_launch_options_do_not_change_results_4096(opts, cuda_device):
     """The production image size (eight-wave four-step column transforms)."""
     from leopard_em_amd.backend import MatchPlan, raw_statistics
@@ -240,7 +242,7 @@ def test_launch_options_do_not_change_results_4096(opts, cuda_device):
     wf = torch.rand((h, h // 2 + 1), generator=gen)
     eul = (torch.rand((4, 3), generator=gen) * torch.tensor([360.0, 180.0, 360.0])).to(cuda_device)
     out = []
-    for o in ({"colpass_twiddles": 0}, opts):
+    for o in ({"colpass_twiddles": 0, "colpass_store": 0, "colpass_persistent": 0}, opts):
         plan = MatchPlan(H, W, h, 2, D, 1, cuda_device)
         for k, v in o.items():
             plan.set_option(k, v)
