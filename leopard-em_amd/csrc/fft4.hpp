@@ -334,7 +334,11 @@ __device__ __forceinline__ void fs_nat2perm_n(float2 (&v)[NT][8], float* const (
   for (int n = 0; n < NT; ++n) dft<8, INV>(v[n]);
 }
 
-template <int N, bool INV, int NT, bool IL = false, class TWT>
+// TAIL = false leaves out the closing barrier after the E1 loads.  A caller may do that
+// when the next LDS writes of the transform area are the next transform's E1 stores:
+// thread t stores exactly the addresses it loaded here (row k1, column t), and every
+// other write (the wave-local E2 / E3 rows) comes after that transform's E1 barrier.
+template <int N, bool INV, int NT, bool IL = false, bool TAIL = true, class TWT>
 __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (&re)[NT], float* const (&im)[NT],
                                               int t, TWT tw) {
   using F = FS<N>;
@@ -416,7 +420,7 @@ __device__ __forceinline__ void fs_perm2nat_n(float2 (&v)[NT][8], float* const (
   for (int n = 0; n < NT; ++n)
 #pragma unroll
     for (int k1 = 0; k1 < 8; ++k1) v[n][k1] = L.ld(n, k1 * RS + t);
-  lds_barrier();
+  if constexpr (TAIL) lds_barrier();
   // stage 1^T: twiddle W_N^{k1 t}, radix-8 over k1 -> i
   {
     float2 w1[8];

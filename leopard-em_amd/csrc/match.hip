@@ -409,7 +409,12 @@ __device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // TS (two columns per block, one correlation per iteration): the result goes through LDS
 // so that every thread stores whole rows (both columns, 16 bytes) and 4 consecutive lanes
 // fill one 64-byte tile -- full-tile stores instead of 8-byte stores into tile halves.
-// xs = the block's LDS (the two columns' transform areas, >= 2 * HH float2).
+// xs = 2 * HH float2 of LDS apart from the transform areas.
+// Barriers per iteration: the forward and inverse E1 exchanges and one before the
+// transposed reads.  The inverse transform's closing barrier is left out (its E1 loads
+// are followed by the next forward E1 stores to the same addresses by the same thread),
+// and xs needs no barrier after its reads: the next writes to xs come after the next
+// iteration's two E1 barriers, which every wave reaches only after its reads.
 template <int HH, int NZJ, int NT, int NX, bool FULL, bool TS, class TWT>
 __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
                                              int hs, int G, int Gs, int g, int c, int t, int k, int klast, int ystore,
@@ -433,7 +438,7 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
     for (int i = 0; i < V; ++i) v[n][i] = c_mulc(xi[i], v[n][i]);
     if (dc) v[n][0] = make_float2(0.f, 0.f);
   }
-  fs_perm2nat_n<HH, true, NT, true>(v, re, im, t, tw);
+  fs_perm2nat_n<HH, true, NT, true, false>(v, re, im, t, tw);
   if constexpr (TS) {
     static_assert(NT == 1, "transposed stores take one correlation per iteration");
     // column c's rows into plane c (lane-consecutive rows: conflict free), then each
@@ -449,7 +454,6 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
       const float2 a = xs[y], b = xs[HH + y];
       if (FULL || y < ystore) gout[((long)(y / TR) * Gs) * TR + (y % TR)] = make_float4(a.x, a.y, b.x, b.y);
     }
-    lds_barrier();  // the next iteration's exchanges reuse the planes
   } else {
     // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
     // tile rows are Gs tiles apart
@@ -486,7 +490,9 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   static_assert(TWM == 0 || FOUR, "the LDS twiddle table serves the four-step path");
   // TWM == 1: lane-ordered copy of the four-step twiddle bases (FsTwLayout) after the
   // transforms' work areas
-  __shared__ float lds[CWB * NT * 2 * LF + (TWM == 1 ? FsTwLayout<HF>::floats : 0)];
+  // TS: the transposed-store planes (2 HH float2) after both
+  constexpr int XS0 = CWB * NT * 2 * LF + (TWM == 1 ? FsTwLayout<HF>::floats : 0);
+  __shared__ float lds[XS0 + (TS ? 4 * HH : 0)];
   // block -> (tile column g, column c): the blocks that write the same 128-byte lines of
   // Gb -- tile columns 2m and 2m+1 (and both columns of each with one column per block)
   // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
@@ -526,8 +532,8 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   }();
   if constexpr (CWB == CW) c = cl;
   float* base = lds + cl * NT * 2 * LF;
-  float2* lds2 = reinterpret_cast<float2*>(lds);
-  static_assert(!TS || (CWB == CW && NT == 1 && FOUR && 2 * HH <= CW * LF), "transposed stores: layout");
+  float2* lds2 = reinterpret_cast<float2*>(lds + XS0);
+  static_assert(!TS || (CWB == CW && NT == 1 && FOUR), "transposed stores: layout");
   float2 xi[V];
   // image column g (permuted order on the four-step path)
   auto load_image = [&](int gg) {
@@ -1476,7 +1482,8 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   } while (0)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
-    constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF) * 4 <= TDTM_LDS_BYTES;     \
+    constexpr int xs_ = (CWB_ == CW && NT_ == 1) ? 4 * HH : 0; /* transposed-store planes */ \
+    constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF + xs_) * 4 <= TDTM_LDS_BYTES;   \
     constexpr bool regs_ = FOUR && NT_ == 1;                                              \
     if (rtw && regs_) {                                                                   \
       if (full) TDTM_COLPASS_T(CWB_, NT_, (regs_ ? 2 : 0), true);                         \
