@@ -1352,6 +1352,8 @@ struct tdtm_match_plan {
                // unpadded image inside it: tdtm_match_plan_set_image_size)
   int Gs;  // tile-column stride of the intermediate (G rounded up to even: 128-byte lines)
   int num_cu = 256;  // compute units of the plan's device
+  const void* occ_kern = nullptr;  // last column-pass kernel whose occupancy was queried
+  int occ_per_cu = 1;              // its resident blocks per CU
   // launch-shape options (tdtm_match_plan_set_option); defaults chosen by measurement
   int opt_colpass_columns = 2;  // columns per column-pass block: 2 or 1
   int opt_colpass_pair = 0;     // two correlations per column-pass iteration
@@ -1507,12 +1509,21 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
 #undef TDTM_COLPASS
 #undef TDTM_COLPASS_T
   const int threads = cols * (HH / V);
+  // resident blocks per CU from the runtime's occupancy calculator (cached per kernel:
+  // the launch shape of a plan's column pass does not change between steps)
+  auto blocks_per_cu = [&]() {
+    if (p->occ_kern != (const void*)kern) {
+      int n = 1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, threads, 0) != hipSuccess || n < 1) n = 1;
+      p->occ_kern = (const void*)kern;
+      p->occ_per_cu = n;
+    }
+    return p->occ_per_cu;
+  };
   if (FOUR && cols == CW && nt == 1 && p->opt_colpass_persistent) {
     // persistent schedule: as many blocks as are resident at once (a multiple of 16,
     // pairs b, b + 8), each pair taking an equal share of the ceil(G/2) x K items
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1)
-      per_cu = 1;
+    const int per_cu = blocks_per_cu();
     long nblk = ((long)per_cu * p->num_cu) / 16 * 16;
     if (nblk < 16) nblk = 16;
     const long total = (long)((p->G + 1) / 2) * K;
@@ -1528,10 +1539,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   // many short blocks; resident blocks per CU from the runtime's occupancy calculator
   int nchunk = p->opt_colpass_chunks;
   if (nchunk <= 0) {
-    int per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0) != hipSuccess || per_cu < 1)
-      per_cu = 1;
-    const long slots = (long)per_cu * p->num_cu;
+    const long slots = (long)blocks_per_cu() * p->num_cu;
     long best = -1;
     for (int n = 1; n <= 16 && n <= K; ++n) {
       int kc = (K + n - 1) / n;
