@@ -214,12 +214,12 @@ def bench_refine(args) -> dict:
     }
 
 
-def read_traffic(path: str):
-    """HBM bytes per step from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
+def read_traffic(path: str, key: str = "hbm_bytes_per_step"):
+    """HBM bytes (``key``) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     try:
         with open(path, encoding="utf-8") as f:
             for ln in f:
-                if ln.startswith("hbm_bytes_per_step:"):
+                if ln.startswith(key + ":"):
                     return int(ln.split(":", 1)[1].strip())
     except OSError:
         return None
@@ -338,7 +338,19 @@ def main():
         print("[bench] phase ms/step: " + ", ".join(f"{k} {v:.3f}" for k, v in phases.items()),
               file=sys.stderr, flush=True)
 
-    traffic = read_traffic(TRAFFIC_SUMMARY) if (H, W, h, B, D) == (4096, 4096, 512, 8, 13) else None
+    measured_shape = (H, W, h, B, D) == (4096, 4096, 512, 8, 13)
+    traffic = read_traffic(TRAFFIC_SUMMARY) if measured_shape else None
+    # the dominant kernel on its own: k_colpass reads the row spectra (h rows per
+    # correlation) and the image spectrum, and writes the half-spectrum intermediate
+    # (all grid rows), W/2+1 complex64 columns each; its time from the same phase events
+    dominant = None
+    if phases:
+        wr = plan.Wg // 2 + 1
+        cp_bytes = (B * D * h * wr + plan.Hg * wr + B * D * plan.Hg * wr) * 8
+        cp_gbs = cp_bytes / (phases["colpass"] / 1e3) / 1e9
+        dominant = {"kernel": "k_colpass", "ms": phases["colpass"], "algorithmic_bytes": cp_bytes,
+                    "achieved": cp_gbs, "frac": cp_gbs / HBM_PEAK_GBS,
+                    "traffic": read_traffic(TRAFFIC_SUMMARY, "colpass_bytes_per_launch") if measured_shape else None}
     ref_rate = REFERENCE_H100_CORR_PER_S.get((H, W))
 
     prep = None
@@ -392,6 +404,7 @@ def main():
             "traffic": traffic,
             "algorithmic_bytes_per_step": ab * B * D,
             "phases_ms_per_step": phases,
+            "dominant_kernel": dominant,
         },
         "cpu_baseline": cpu,
         "preparation": prep,
