@@ -195,18 +195,21 @@ __global__ __launch_bounds__(RPB*(HS / 2) / V) void k_proj_rows(const float2* __
   constexpr int T = M / V;
   constexpr int WR = M + 1;
   constexpr int LF = LdsSize<M>::floats;
-  __shared__ float lds[RPB * 2 * LF];
-  __shared__ float2 stg[RPB * WR];
+  // one region per row: first the staged half-spectrum row, then the transform's split
+  // re / im arrays (the row's own wave reads the staged row before it writes them)
+  __shared__ __attribute__((aligned(16))) float lds[RPB * 2 * LF];
+  static_assert(2 * WR <= 2 * LF, "a staged row fits its transform region");
   const int rl = threadIdx.x / T, t = threadIdx.x % T;
   const int y = blockIdx.x * RPB + rl;
   const int k = blockIdx.y;
   const float2* row = Tb + ((long)k * HS + y) * WR;
   static_assert(T <= 64 && 64 % T == 0, "a row's threads lie in one wave");
   // every exchange below stays inside the row's threads, i.e. inside one wave
-  for (int e = t; e < WR; e += T) stg[rl * WR + e] = row[e];
+  float2* stg = reinterpret_cast<float2*>(lds + rl * 2 * LF);
+  for (int e = t; e < WR; e += T) stg[e] = row[e];
   wave_lds_order();
   float2 v[V];
-  const float2* g = stg + rl * WR;
+  const float2* g = stg;
 #pragma unroll
   for (int i = 0; i < V; ++i) {
     const int m = t + i * T;
