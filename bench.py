@@ -229,8 +229,9 @@ def read_traffic(path: str, key: str = "hbm_bytes_per_step"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    # untimed steps first: the clocks settle within a few steps
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=8, help="orientations per step")
     ap.add_argument("--size", type=int, default=4096, help="image edge (H = W)")
     ap.add_argument("--image", default=None,
