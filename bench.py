@@ -34,7 +34,12 @@ def algorithmic_bytes_per_correlation(H: int, W: int) -> int:
     return 2 * 8 * H * (W // 2 + 1)
 
 
-def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
+def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0, return_micrograph=False):
+    """Config-2 style synthetic search inputs built on ``device``: Gaussian-blob template,
+    micrograph of embedded CTF-filtered projections at SNR 0.05 with per-particle defocus,
+    whitened image spectrum, template rfft and the 13-plane CTF stack.  With
+    ``return_micrograph`` the SyntheticMicrograph (true poses, centres, defocus) is
+    appended to the returned tuple."""
     import torch
 
     from leopard_em_amd import synthetic
@@ -49,6 +54,8 @@ def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0):
     img_dft, wf = pp.prepare_image(image, h)
     dfs = torch.arange(-1200.0, 1201.0, 200.0)
     ctf = pp.ctf_filter_stack((h, h), 12200.0, 11800.0, 30.0, dfs, torch.tensor([0.0]), device=device)
+    if return_micrograph:
+        return img_dft, tdft, ctf, wf.to(device), dfs, vol, image, mg
     return img_dft, tdft, ctf, wf.to(device), dfs, vol, image
 
 

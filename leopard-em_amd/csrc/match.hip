@@ -1780,6 +1780,10 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
       return fail("rowpass_staging 1 / 2 leave no LDS for rowpass_twiddles = 1");
     if ((value == 1 || value == 2) && p->opt_rowpass_rows != 4)
       return fail("rowpass_staging 1 / 2 need rowpass_rows = 4");
+    // the asynchronous copy loop is written for whole waves: a row-pass block of W / 4
+    // threads is below one wave for W < 256 (and rows above 4096 always stage synchronously)
+    if ((value == 1 || value == 2) && (p->W < 256 || p->W > 4096))
+      return fail("rowpass_staging 1 / 2 need 256 <= W <= 4096 (a row-pass block of at least one wave)");
     p->opt_rowpass_staging = value;
   } else if (k == "rowpass_rows") {
     if (value != 1 && value != 2 && value != 4) return fail("rowpass_rows must be 1, 2 or 4");
@@ -1834,6 +1838,8 @@ int tdtm_stats_merge(int64_t n, float* mip, int32_t* idx, float* shi, float* slo
                      const float* mip_b, const int32_t* idx_b, const float* shi_b, const float* slo_b,
                      const float* qhi_b, const float* qlo_b, void* stream) {
   if (n <= 0) return fail("n must be positive");
+  if (!mip || !idx || !shi || !slo || !qhi || !qlo || !mip_b || !idx_b || !shi_b || !slo_b || !qhi_b || !qlo_b)
+    return fail("null argument");
   k_stats_merge<<<(n + 255) / 256, 256, 0, (hipStream_t)stream>>>(n, mip, idx, shi, slo, qhi, qlo, mip_b, idx_b,
                                                                  shi_b, slo_b, qhi_b, qlo_b);
   TDTM_LAUNCHED();
