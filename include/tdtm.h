@@ -236,6 +236,21 @@ int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, i
                        void* image_dft, float* whitening_template, void* workspace, void* stream);
 
 /*
+ * tdtm_prepare_image without any device allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same outputs as tdtm_prepare_image,
+ * but the twiddle table and the workspace live in one static device array per GPU of at
+ * most TDTM_PREPARE_NOALLOC_SCRATCH_BYTES, held until the process ends; the call makes no
+ * hipMalloc / hipFree.  Limits: H and W powers of two in [32, 4096] x [32, 8192] (the
+ * direct-DFT path of other sizes needs an image-sized workspace) and at most 8192
+ * frequency bins; larger requests fail with an error.  Every call on a GPU uses that
+ * GPU's one array: calls that overlap on one GPU (two streams, two threads) race.
+ */
+#define TDTM_PREPARE_NOALLOC_SCRATCH_BYTES (320 * 1024)
+int tdtm_prepare_image_noalloc(const float* image, int H, int W, int h, int whitening, int num_freq_bins,
+                               double max_freq, void* image_dft, float* whitening_template, void* stream);
+
+/*
  * CTF x B-envelope filter stack, out float32 (num_pixel, num_defocus, h, h/2+1):
  * replaces calculate_ctf_filter_stack_full_args (src/leopard_em/utils/ctf_utils.py:19-116)
  * with torch_ctf's sign convention (CTF(0) = +amplitude contrast).  defocus = offsets +

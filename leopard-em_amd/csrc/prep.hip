@@ -508,11 +508,15 @@ int launch_cols(int n, float2* data, int ncols, long estride, long bstride, int 
   }
 }
 
+int rfft2_tw(const float* image, int H, int W, float2* out, const float2* tw, hipStream_t st) {
+  if (launch_rows<false>(W, image, H, tw, out, st)) return 1;
+  return launch_cols<false>(H, out, W / 2 + 1, W / 2 + 1, 0, 1, tw, st);
+}
+
 int rfft2_impl(const float* image, int H, int W, float2* out, hipStream_t st) {
   const float2* tw = nullptr;
   if (device_twiddles(&tw)) return 1;
-  if (launch_rows<false>(W, image, H, tw, out, st)) return 1;
-  return launch_cols<false>(H, out, W / 2 + 1, W / 2 + 1, 0, 1, tw, st);
+  return rfft2_tw(image, H, W, out, tw, st);
 }
 
 struct PrepWorkspace {
@@ -546,6 +550,115 @@ PrepWorkspace carve(void* ws, int H, int W, int requested_bins) {
   const long used = 3L * w.nb + w.npart;
   w.dft = reinterpret_cast<double2*>(p + ((used + 1) & ~1L));  // 16-byte aligned
   return w;
+}
+
+// The preparation chain on an existing twiddle table and workspace (shared by the
+// allocating entry point and the scratch-array one).
+int prepare_image_impl(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
+                       float2* X, float* whitening_template, void* workspace, const float2* tw, hipStream_t st) {
+  const PrepWorkspace w = carve(workspace, H, W, num_freq_bins);
+  const double fm = max_freq;
+  if (prep_pow2(H, W)) {
+    if (rfft2_tw(image, H, W, X, tw, st)) return 1;
+  } else {  // direct double-precision DFTs for sizes the FFT kernels do not take
+    k_dft_rows_fwd<<<H, DFT_BLOCK, 0, st>>>(image, W, w.dft);
+    TDTM_LAUNCHED();
+    k_dft_cols_fwd<<<W / 2 + 1, DFT_BLOCK, 0, st>>>(w.dft, H, W / 2 + 1, X);
+    TDTM_LAUNCHED();
+  }
+  k_zero_dc<<<1, 1, 0, st>>>(X);
+  TDTM_LAUNCHED();
+  if (whitening) {
+    k_radial_power<<<w.nb, 256, 0, st>>>(X, H, W, w.nb, fm, w.sums, w.cnts);
+    TDTM_LAUNCHED();
+    k_whiten_curve<<<1, 1024, 0, st>>>(w.sums, w.cnts, w.nb, w.curve);
+    TDTM_LAUNCHED();
+  } else {
+    k_fill_ones<<<(w.nb + 255) / 256, 256, 0, st>>>(w.curve, w.nb);  // flat curve: every filter value is 1
+    TDTM_LAUNCHED();
+  }
+  const long nt = (long)h * (h / 2 + 1);
+  k_whiten_eval<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(w.curve, w.nb, fm, h, h, whitening_template);
+  TDTM_LAUNCHED();
+  k_prep_filter<<<w.npart, FILTER_BLOCK, 0, st>>>(X, H, W, w.curve, w.nb, fm, w.part);
+  TDTM_LAUNCHED();
+  k_prep_scale<<<w.npart, FILTER_BLOCK, 0, st>>>(X, (long)H * (W / 2 + 1), w.part, w.npart, (double)H * (double)W);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+int prepare_image_checks(const float* image, int H, int W, int h, int num_freq_bins, double max_freq,
+                         const void* image_dft, const float* whitening_template) {
+  if (!image || !image_dft || !whitening_template) return fail("null argument");
+  if (!prep_size_ok(H, W))
+    return fail("image size must be H <= 4096 by an even W <= 8192, got " + std::to_string(H) + "x" +
+                std::to_string(W));
+  if (h < 2 || (h & 1)) return fail("template size must be even and >= 2");
+  if (num_freq_bins == 1 || num_freq_bins > 65536) return fail("num_freq_bins must be 0 (default) or in [2, 65536]");
+  if (!(max_freq > 0.0)) return fail("max_freq must be positive");
+  return 0;
+}
+
+// ------------------------------------------------- preparation without allocation
+// tdtm_prepare_image_noalloc keeps its twiddle table and its workspace (per-bin power
+// sums and counts, the whitening curve, per-block energy partials) in `scratch`, one
+// static device array per GPU that lives until the process ends.  Host code reaches it
+// only through the address hipGetSymbolAddress returns for the current device (the
+// array's name on the host is not a device address).  Every call rewrites all of it
+// before reading it: the twiddle table first, then each bin / block slot is written
+// by one block before the next kernel reads it, so nothing has to start at zero.  Calls
+// that overlap on one GPU share the array and race.
+constexpr int NOALLOC_MAX_BINS = 8192;
+constexpr long NOALLOC_TW_BYTES = (long)TW_N * sizeof(float2);  // 64 KiB, 256-byte aligned
+constexpr long NOALLOC_WS_DOUBLES = 3L * NOALLOC_MAX_BINS + (4096L * 4097 + FILTER_BLOCK * FILTER_PER_THREAD - 1) /
+                                                                (FILTER_BLOCK * FILTER_PER_THREAD);
+constexpr long NOALLOC_SCRATCH_BYTES = NOALLOC_TW_BYTES + ((NOALLOC_WS_DOUBLES * 8 + 255) / 256) * 256;
+static_assert(NOALLOC_SCRATCH_BYTES <= TDTM_PREPARE_NOALLOC_SCRATCH_BYTES, "the scratch array stays within the size the header states");
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratch_mutex;
+void* g_scratch_dev[MAX_DEVICES] = {};
+
+// Device address of this GPU's scratch array (looked up once per device).
+int scratch_address(void** out) {
+  int dev = 0;
+  TDTM_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
+
+// The twiddle table of twiddle_table() (common.hpp), built in double on the device by the
+// same first-octant mapping (exact quarter / eighth turns).
+__global__ void k_twiddle_fill(float2* __restrict__ tw) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= TW_N) return;
+  constexpr int Q = TW_N / 4;
+  const int q = m / Q, r = m % Q;
+  double c, sn;
+  if (2 * r <= Q) {
+    const double a = 2.0 * M_PI * (double)r / (double)TW_N;
+    c = cos(a);
+    sn = sin(a);
+  } else {
+    const double a = 2.0 * M_PI * (double)(Q - r) / (double)TW_N;
+    c = sin(a);
+    sn = cos(a);
+  }
+  double cq = c, sq = sn;
+  if (q == 1) cq = -sn, sq = c;
+  if (q == 2) cq = -c, sq = -sn;
+  if (q == 3) cq = sn, sq = -c;
+  tw[m] = make_float2((float)cq, (float)-sq);
 }
 
 }  // namespace
@@ -608,44 +721,33 @@ int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins) {
 
 int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
                        void* image_dft, float* whitening_template, void* workspace, void* stream) {
-  if (!image || !image_dft || !whitening_template || !workspace) return fail("null argument");
-  if (!prep_size_ok(H, W))
-    return fail("image size must be H <= 4096 by an even W <= 8192, got " + std::to_string(H) + "x" +
-                std::to_string(W));
-  if (h < 2 || (h & 1)) return fail("template size must be even and >= 2");
-  if (num_freq_bins == 1 || num_freq_bins > 65536) return fail("num_freq_bins must be 0 (default) or in [2, 65536]");
-  if (!(max_freq > 0.0)) return fail("max_freq must be positive");
+  if (prepare_image_checks(image, H, W, h, num_freq_bins, max_freq, image_dft, whitening_template)) return 1;
+  if (!workspace) return fail("null argument");
+  const float2* tw = nullptr;
+  if (prep_pow2(H, W) && device_twiddles(&tw)) return 1;
+  return prepare_image_impl(image, H, W, h, whitening, num_freq_bins, max_freq, static_cast<float2*>(image_dft),
+                            whitening_template, workspace, tw, static_cast<hipStream_t>(stream));
+}
+
+int tdtm_prepare_image_noalloc(const float* image, int H, int W, int h, int whitening, int num_freq_bins,
+                               double max_freq, void* image_dft, float* whitening_template, void* stream) {
+  if (prepare_image_checks(image, H, W, h, num_freq_bins, max_freq, image_dft, whitening_template)) return 1;
+  if (!prep_pow2(H, W) || H > 4096 || W > 8192)
+    return fail("tdtm_prepare_image_noalloc: image edges must be powers of two in [32, 4096] x [32, 8192], got " +
+                std::to_string(H) + "x" + std::to_string(W));
+  if (num_bins(H, W, num_freq_bins) > NOALLOC_MAX_BINS)
+    return fail("tdtm_prepare_image_noalloc: at most " + std::to_string(NOALLOC_MAX_BINS) + " frequency bins");
+  if (tdtm_prepare_image_workspace_bytes(H, W, num_freq_bins) > NOALLOC_SCRATCH_BYTES - NOALLOC_TW_BYTES)
+    return fail("tdtm_prepare_image_noalloc: workspace exceeds the scratch array");
+  void* base = nullptr;
+  if (scratch_address(&base)) return 1;
   const hipStream_t st = static_cast<hipStream_t>(stream);
-  float2* X = static_cast<float2*>(image_dft);
-  const PrepWorkspace w = carve(workspace, H, W, num_freq_bins);
-  const double fm = max_freq;
-  if (prep_pow2(H, W)) {
-    if (rfft2_impl(image, H, W, X, st)) return 1;
-  } else {  // direct double-precision DFTs for sizes the FFT kernels do not take
-    k_dft_rows_fwd<<<H, DFT_BLOCK, 0, st>>>(image, W, w.dft);
-    TDTM_LAUNCHED();
-    k_dft_cols_fwd<<<W / 2 + 1, DFT_BLOCK, 0, st>>>(w.dft, H, W / 2 + 1, X);
-    TDTM_LAUNCHED();
-  }
-  k_zero_dc<<<1, 1, 0, st>>>(X);
+  float2* tw = static_cast<float2*>(base);
+  k_twiddle_fill<<<TW_N / 256, 256, 0, st>>>(tw);
   TDTM_LAUNCHED();
-  if (whitening) {
-    k_radial_power<<<w.nb, 256, 0, st>>>(X, H, W, w.nb, fm, w.sums, w.cnts);
-    TDTM_LAUNCHED();
-    k_whiten_curve<<<1, 1024, 0, st>>>(w.sums, w.cnts, w.nb, w.curve);
-    TDTM_LAUNCHED();
-  } else {
-    k_fill_ones<<<(w.nb + 255) / 256, 256, 0, st>>>(w.curve, w.nb);  // flat curve: every filter value is 1
-    TDTM_LAUNCHED();
-  }
-  const long nt = (long)h * (h / 2 + 1);
-  k_whiten_eval<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(w.curve, w.nb, fm, h, h, whitening_template);
-  TDTM_LAUNCHED();
-  k_prep_filter<<<w.npart, FILTER_BLOCK, 0, st>>>(X, H, W, w.curve, w.nb, fm, w.part);
-  TDTM_LAUNCHED();
-  k_prep_scale<<<w.npart, FILTER_BLOCK, 0, st>>>(X, (long)H * (W / 2 + 1), w.part, w.npart, (double)H * (double)W);
-  TDTM_LAUNCHED();
-  return 0;
+  void* ws = static_cast<unsigned char*>(base) + NOALLOC_TW_BYTES;
+  return prepare_image_impl(image, H, W, h, whitening, num_freq_bins, max_freq, static_cast<float2*>(image_dft),
+                            whitening_template, ws, tw, st);
 }
 
 }  // extern "C"

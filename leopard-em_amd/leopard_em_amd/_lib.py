@@ -45,6 +45,8 @@ SIGNATURES = {
     "tdtm_prepare_image_workspace_bytes": (c_int64, [c_int, c_int, c_int]),
     "tdtm_prepare_image": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.c_double, c_void_p, c_void_p,
                                    c_void_p, c_void_p]),
+    "tdtm_prepare_image_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.c_double, c_void_p,
+                                           c_void_p, c_void_p]),
     "tdtm_ctf_filter_stack": (c_int, [c_int, c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
