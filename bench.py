@@ -221,6 +221,68 @@ def bench_refine(args) -> dict:
     }
 
 
+def kernel_sources_digest() -> str:
+    """SHA-256 over the HIP sources of libtdtm (names and contents, sorted by name)."""
+    import hashlib
+
+    csrc = os.path.join(REPO, "leopard-em_amd", "csrc")
+    h = hashlib.sha256()
+    for name in sorted(os.listdir(csrc)):
+        if name.endswith((".hip", ".hpp")):
+            h.update(name.encode())
+            with open(os.path.join(csrc, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
+
+
+def read_summary_field(path: str, key: str):
+    try:
+        with open(path, encoding="utf-8") as f:
+            for ln in f:
+                if ln.startswith(key + ":"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        return None
+    return None
+
+
+def measure_recall(plan, eul, dfs, total, mg, h, n_true, zcut, radius=2.0):
+    """Embedded particles found by the search just run: z = (MIP - mean) / std over the
+    timed orientations, peaks above ``zcut`` on the valid crop picked with the reference's
+    greedy 5-pixel suppression (analysis/zscore_metric.py:42-83); a particle whose true
+    pose is among the orientations counts as found when a peak lies within ``radius``
+    pixels of its template top-left (centre - h//2)."""
+    import numpy as np
+    import torch
+
+    from leopard_em_amd.analysis import find_peaks_from_zscore
+
+    dvals = dfs.to(plan.device, torch.float32).contiguous()
+    maps = plan.finalize(eul, dvals, total)
+    vy, vx = plan.H - h + 1, plan.W - h + 1
+    z = maps["scaled_mip"][:vy, :vx].cpu()
+    py, px = find_peaks_from_zscore(z, zcut)
+    peaks = np.stack([py.numpy(), px.numpy()], 1).astype(float) if len(py) else np.zeros((0, 2))
+    tl = np.rint(mg.centers_yx[:n_true]).astype(int) - h // 2
+    inside = (tl[:, 0] >= 0) & (tl[:, 0] < vy) & (tl[:, 1] >= 0) & (tl[:, 1] < vx)
+    tl = tl[inside]
+    found = 0
+    matched = np.zeros(len(peaks), dtype=bool)
+    for t in tl:
+        if len(peaks):
+            d = np.hypot(peaks[:, 0] - t[0], peaks[:, 1] - t[1])
+            hit = d <= radius
+            if hit.any():
+                found += 1
+                matched |= hit
+    zt = [float(z[t[0], t[1]]) for t in tl]
+    return {"zcut": zcut, "particles": int(len(tl)), "found": int(found),
+            "recall": found / len(tl) if len(tl) else None, "false_peaks": int((~matched).sum()),
+            "median_particle_z": float(np.median(zt)) if zt else None,
+            "rule": f"z > {zcut} peak within {radius} px of the true template top-left; the true poses are in the "
+                    "timed orientation slice"}
+
+
 def read_traffic(path: str, key: str = "hbm_bytes_per_step"):
     """HBM bytes (``key``) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     try:
@@ -236,7 +298,8 @@ def read_traffic(path: str, key: str = "hbm_bytes_per_step"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    # 512 steps of 8 orientations: the 4096-orientation slice of the production grid
+    ap.add_argument("--steps", type=int, default=512)
     # untimed steps first: the clocks settle within a few steps
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=8, help="orientations per step")
@@ -251,6 +314,8 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run one device copy of this many MiB first (reference kernel for HBM byte counters)")
     ap.add_argument("--no-phase-timing", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="time plain step launches instead of graph replays")
+    ap.add_argument("--zcut", type=float, default=7.5, help="z-score cut-off of the recall measurement")
     ap.add_argument("--no-prep-timing", action="store_true",
                     help="skip timing the input preparation kernels (synthetic inputs only)")
     ap.add_argument("--workload", choices=("match", "refine"), default="match",
@@ -279,8 +344,10 @@ def main():
     print(f"[bench] building {args.inputs} inputs {H}x{W}, template {h}^3", file=sys.stderr, flush=True)
     if args.inputs == "synthetic":
         n_blobs = 5000 if h == 512 else max(100, int(5000 * (h / 512) ** 3))
-        img_dft, tdft, ctf, wf, dfs, vol, image = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200)
+        img_dft, tdft, ctf, wf, dfs, vol, image, mg = make_inputs(dev, H, W, h, n_blobs, 200.0 * h / 512, 200,
+                                                                  return_micrograph=True)
     else:
+        mg = None
         gen = torch.Generator(device=dev).manual_seed(0)
         img_dft = torch.randn((H, W // 2 + 1), dtype=torch.complex64, device=dev, generator=gen)
         tdft = torch.randn((h, h, h // 2 + 1), dtype=torch.complex64, device=dev, generator=gen)
@@ -289,12 +356,19 @@ def main():
         wf = torch.rand((h, h // 2 + 1), device=dev, generator=gen)
     D = dfs.shape[0]
     B = args.batch
-    n_orient = (args.warmup + 2 * args.steps) * B
-    # a strided slice of the production grid (out-of-plane 2.5 deg, in-plane 1.5 deg)
+    n_timed = args.steps * B
+    # orientations: the embedded particles' true poses first (synthetic inputs), then a
+    # strided slice of the production grid (out-of-plane 2.5 deg, in-plane 1.5 deg); the
+    # warm-up steps take other grid orientations
     grid = get_uniform_euler_angles(psi_step=1.5, theta_step=2.5)
     full_grid = grid.shape[0]
-    stride = max(1, full_grid // max(n_orient, 4096))
-    eul = grid[::stride][:max(n_orient, 4096)].contiguous().to(dev)
+    n_grid = n_timed + args.warmup * B
+    stride = max(1, full_grid // max(n_grid, 1))
+    strided = grid[::stride][:n_grid]
+    true_poses = torch.tensor(mg.eulers, dtype=torch.float32) if mg is not None else torch.zeros((0, 3))
+    n_true = min(true_poses.shape[0], n_timed)
+    eul = torch.cat([true_poses[:n_true], strided[:n_timed - n_true]]).contiguous().to(dev)
+    eul_warm = strided[n_timed - n_true:][:max(args.warmup * B, 1)].contiguous().to(dev)
 
     if args.copy_calibration_mib:
         nbytes = args.copy_calibration_mib << 20
@@ -307,39 +381,67 @@ def main():
         del a, b
 
     plan = MatchPlan(H, W, h, B, D, 1, dev)
+    plan.set_option("graph", 0 if args.no_graph else 1)
     plan.set_inputs(img_dft, ctf, wf)
     plan.reset_stats()
     stream = torch.cuda.current_stream(dev)
-    for i in range(args.warmup):
-        plan.step(tdft, eul, i * B, B)
+    plan.run(tdft, eul_warm, 0, min(args.warmup * B, eul_warm.shape[0]))
+    torch.cuda.synchronize(dev)
+    plan.reset_stats()
     torch.cuda.synchronize(dev)
     print("[bench] warm-up done", file=sys.stderr, flush=True)
 
+    # timed region: the whole orientation slice through tdtm_match_run (full batches
+    # replayed from the captured step graph unless --no-graph)
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     ev0.record(stream)
-    for i in range(args.steps):
-        plan.step(tdft, eul, (args.warmup + i) * B, B)
+    plan.run(tdft, eul, 0, n_timed)
     ev1.record(stream)
     torch.cuda.synchronize(dev)
     wall = time.perf_counter() - t0
     gpu_ms = ev0.elapsed_time(ev1)
     ms_per_step = gpu_ms / args.steps
-    corr = args.steps * B * D
+    corr = n_timed * D
     value = corr / (gpu_ms / 1e3)
     ab = algorithmic_bytes_per_correlation(H, W)
     achieved = ab * B * D / (ms_per_step / 1e3) / 1e9
-    print(f"[bench] {args.steps} steps: {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms), {value:.1f} corr/s",
+    print(f"[bench] {args.steps} steps ({n_timed} orientations): {gpu_ms:.1f} ms (wall {wall * 1e3:.1f} ms), "
+          f"{value:.1f} corr/s", file=sys.stderr, flush=True)
+
+    recall = None
+    if mg is not None:
+        recall = measure_recall(plan, eul, dfs, n_timed * D, mg, h, n_true, args.zcut)
+        print(f"[bench] recall at z > {args.zcut}: {recall['found']} of {recall['particles']} embedded particles, "
+              f"{recall['false_peaks']} other peaks", file=sys.stderr, flush=True)
+
+    # the same batches launched as plain steps (no graph), for the per-step launch cost
+    n_cmp = min(args.steps, 64)
+    plan.set_option("graph", 0)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    plan.run(tdft, eul, 0, n_cmp * B)
+    e1.record(stream)
+    e1.synchronize()
+    ms_direct = e0.elapsed_time(e1) / n_cmp
+    plan.set_option("graph", 1)
+    e0.record(stream)
+    plan.run(tdft, eul, 0, n_cmp * B)
+    e1.record(stream)
+    e1.synchronize()
+    ms_graph = e0.elapsed_time(e1) / n_cmp
+    print(f"[bench] ms/step over {n_cmp} steps: direct launches {ms_direct:.3f}, graph replays {ms_graph:.3f}",
           file=sys.stderr, flush=True)
 
     phases = None
     if not args.no_phase_timing:
-        # same number of steps again, with HIP events between the phases of each step
+        # HIP events between the phases of each step (plain launches)
         plan.set_option("phase_timing", 1)
-        for i in range(args.steps):
-            plan.step(tdft, eul, (args.warmup + args.steps + i) * B, B)
+        for i in range(n_cmp):
+            plan.step(tdft, eul, i * B, B)
         ph, n = plan.phase_times()
         plan.set_option("phase_timing", 0)
         phases = {k: v / n for k, v in ph.items()}
@@ -347,7 +449,9 @@ def main():
               file=sys.stderr, flush=True)
 
     measured_shape = (H, W, h, B, D) == (4096, 4096, 512, 8, 13)
-    traffic = read_traffic(TRAFFIC_SUMMARY) if measured_shape else None
+    # the committed counter summary counts only if it was measured on these kernel sources
+    traffic_current = read_summary_field(TRAFFIC_SUMMARY, "kernel_sources_sha256") == kernel_sources_digest()
+    traffic = read_traffic(TRAFFIC_SUMMARY) if measured_shape and traffic_current else None
     # the dominant kernel on its own: k_colpass reads the row spectra (h rows per
     # correlation) and the image spectrum, and writes the half-spectrum intermediate
     # (all grid rows), W/2+1 complex64 columns each; its time from the same phase events
@@ -358,7 +462,8 @@ def main():
         cp_gbs = cp_bytes / (phases["colpass"] / 1e3) / 1e9
         dominant = {"kernel": "k_colpass", "ms": phases["colpass"], "algorithmic_bytes": cp_bytes,
                     "achieved": cp_gbs, "frac": cp_gbs / HBM_PEAK_GBS,
-                    "traffic": read_traffic(TRAFFIC_SUMMARY, "colpass_bytes_per_launch") if measured_shape else None}
+                    "traffic": (read_traffic(TRAFFIC_SUMMARY, "colpass_bytes_per_launch")
+                                if measured_shape and traffic_current else None)}
     ref_rate = REFERENCE_H100_CORR_PER_S.get((H, W))
 
     prep = None
@@ -385,6 +490,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
+        "ms_per_step_direct_launch": ms_direct,
+        "ms_per_step_graph_replay": ms_graph,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": value / ref_rate if ref_rate else None,
@@ -398,7 +505,8 @@ def main():
                         f"{B} orientations per step",
             "image": [H, W], "search_grid": [plan.Hg, plan.Wg], "template": h, "defocus": D,
             "orientation_batch": B,
-            "orientations": "strided slice of the 2.5/1.5 deg uniform grid",
+            "orientations": (f"{n_timed} = {n_true} embedded particles' true poses + a strided slice of the "
+                             "2.5/1.5 deg uniform grid"),
             "full_grid_orientations": full_grid,
             "full_grid_hours_extrapolated": full_grid * D / value / 3600.0,
         },
@@ -410,10 +518,14 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "traffic_source": (f"{os.path.relpath(TRAFFIC_SUMMARY, REPO)} (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                               "measured on these kernel sources)" if traffic is not None else
+                               "none: the committed counter summary is for other kernel sources or shapes"),
             "algorithmic_bytes_per_step": ab * B * D,
             "phases_ms_per_step": phases,
             "dominant_kernel": dominant,
         },
+        "recall": recall,
         "cpu_baseline": cpu,
         "preparation": prep,
     }

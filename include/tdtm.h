@@ -79,8 +79,10 @@ int tdtm_match_plan_set_image_size(tdtm_match_plan_t plan, int image_h, int imag
  * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat,
  * 3: synchronous de-interleaved with batched loads; default 3), "rowpass_twiddles" (0: global
  * table, 1: LDS copy, default; needs staging 0 or 3), "rowpass_rows" (rows of a 4-row tile
- * row per row-pass block: 1 (default), 2 or 4; below 4 needs staging 0 or 3), "phase_timing"
- * (0/1: record HIP events between the phases of every step).
+ * row per row-pass block: 1 (default), 2 or 4; below 4 needs staging 0 or 3; staging 1 / 2
+ * need 256 <= W <= 4096), "graph" (0/1: tdtm_match_run replays full-batch steps from a
+ * captured HIP graph; default 1), "phase_timing" (0/1: record HIP events between the
+ * phases of every step; tdtm_match_run then launches steps directly).
  * Results do not depend on the options.
  */
 int tdtm_match_plan_set_option(tdtm_match_plan_t plan, const char* key, int value);
@@ -127,6 +129,20 @@ int tdtm_stats_init(int64_t n, float* mip, int32_t* best_index, float* sum_hi, f
 int tdtm_match_step(tdtm_match_plan_t plan, const void* template_dft, const float* euler_deg,
                     int num_orientations, int start, int count, float* mip, int32_t* best_index,
                     float* sum_hi, float* sum_lo, float* sq_hi, float* sq_lo, void* stream);
+
+/*
+ * Orientations [start, end) in steps of max_orientation_batch: the orientation loop of
+ * _core_match_template_single_gpu (core_match_template.py:492-554) on one device, each
+ * step as tdtm_match_step.  With the plan option "graph" (default 1) the full-batch steps
+ * are replayed from a HIP graph: one step is captured once (its first orientation read
+ * from device memory and advanced on the device) and launched once per batch; the graph
+ * is re-captured when the template, Euler or statistics pointers or num_orientations
+ * change.  A last partial batch runs as a plain step.  Results are bit-identical to
+ * calling tdtm_match_step per batch.
+ */
+int tdtm_match_run(tdtm_match_plan_t plan, const void* template_dft, const float* euler_deg,
+                   int num_orientations, int start, int end, float* mip, int32_t* best_index,
+                   float* sum_hi, float* sum_lo, float* sq_hi, float* sq_lo, void* stream);
 
 /*
  * Final statistics (FP64): mean = sum/N, std = sqrt(max(sumsq/N - mean^2, 0)),

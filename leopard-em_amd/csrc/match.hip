@@ -81,8 +81,11 @@ __device__ __forceinline__ void rotmat_zyz(const float* e, float (&r)[9]) {
 
 // R = Rz(phi) Ry(theta) Rz(psi) of orientation o0 + b, right-multiplied by the rotation
 // of `base` (refinement: offset composed with the particle's pose) when base != nullptr.
+// o0p != nullptr: the first orientation is read from device memory (graph replays of a
+// captured step advance it on the device).
 __global__ void k_slice(const float2* __restrict__ vol, int n, const float* __restrict__ euler, int o0, int nb,
-                        const float* __restrict__ base, float2* __restrict__ S) {
+                        const float* __restrict__ base, float2* __restrict__ S, const int* __restrict__ o0p) {
+  if (o0p != nullptr) o0 = *o0p;
   const int wr = n / 2 + 1;
   const long total = (long)nb * n * wr;
   const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -969,7 +972,8 @@ __global__ __launch_bounds__(RB*(WS / 2) / V, 4) void k_rowpass(const float2* __
                                                             const float2* __restrict__ tw, float* __restrict__ mip,
                                                             int* __restrict__ bidx, float* __restrict__ shi,
                                                             float* __restrict__ slo, float* __restrict__ qhi,
-                                                            float* __restrict__ qlo) {
+                                                            float* __restrict__ qlo, const int* __restrict__ o0p) {
+  if (o0p != nullptr) o0 = *o0p;  // device-side first orientation (graph replays)
   using C = RowCfg<WS>;
   constexpr int T = C::T;
   __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, MODE, RB>::floats + (TWM ? RowTw<WS>::floats : 0)];
@@ -1034,6 +1038,10 @@ __global__ __launch_bounds__(RB*(WS / 2) / V, 4) void k_rowpass(const float2* __
     }
   }
 }
+
+// Device-side first orientation of the next captured step.
+__global__ void k_set_start(int* o0p, int v) { *o0p = v; }
+__global__ void k_advance_start(int* o0p, int step) { *o0p += step; }
 
 __global__ void k_stats_init(long n, float* mip, int* idx, float* a, float* b, float* c, float* d) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1388,6 +1396,17 @@ struct tdtm_match_plan {
   int rslots_cap = 0;
   int64_t bytes = 0;
   bool image_set = false, filters_set = false;
+  // graph replay of full-batch steps (tdtm_match_run with option "graph"): one captured
+  // step reads its first orientation from d_o0 and advances it; the executable is kept
+  // for the arguments it was captured with (graph_key) and re-captured when they change
+  int opt_graph = 1;
+  int* d_o0 = nullptr;
+  const int* o0p_launch = nullptr;  // set while a step is being captured
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::array<const void*, 8> graph_key{};
+  int graph_O = -1;
 };
 
 template <class T>
@@ -1405,8 +1424,20 @@ static void release_phase_events(tdtm_match_plan* p) {
   p->phase_events.clear();
 }
 
+static void release_graph(tdtm_match_plan* p) {
+  if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  p->gexec = nullptr;
+  p->graph = nullptr;
+  p->graph_key = {};
+  p->graph_O = -1;
+}
+
 static void plan_free(tdtm_match_plan* p) {
   release_phase_events(p);
+  release_graph(p);
+  if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
+  if (p->d_o0) (void)hipFree(p->d_o0);
   void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb, p->ctfbuf, p->rpart,
                   p->rslots};
   for (void* b : bufs)
@@ -1578,7 +1609,7 @@ template <int WS, int MODE, int TWM, int RB>
 static void launch_rowpass_rb(tdtm_match_plan* p, int K, int nb, int O, int o0, float* mip, int* idx, float* shi,
                               float* slo, float* qhi, float* qlo, hipStream_t st) {
   k_rowpass<WS, MODE, TWM, RB><<<dim3((p->H / TR) * (TR / RB)), RB * (WS / 2) / V, 0, st>>>(
-      p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi, qlo);
+      p->Gb, p->H, p->G, K, nb, p->D, O, o0, p->tw, mip, idx, shi, slo, qhi, qlo, p->o0p_launch);
 }
 
 template <int WS, int MODE, int TWM>
@@ -1720,6 +1751,7 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   rc |= rc ? 0 : dalloc(&p->Gb, K * H * p->Gs * CW, p->bytes);
   rc |= rc ? 0 : dalloc(&p->ctfbuf, (size_t)C * D * h * p->wr, p->bytes);
   rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR * REFINE_CHUNKS_MAX, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->d_o0, 1, p->bytes);
   if (rc) {
     plan_free(p);
     delete p;
@@ -1795,6 +1827,9 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     if (value && (p->opt_rowpass_staging == 1 || p->opt_rowpass_staging == 2))
       return fail("rowpass_twiddles = 1 needs rowpass_staging 0 or 3");
     p->opt_rowpass_twiddles = value;
+  } else if (k == "graph") {
+    p->opt_graph = value != 0;
+    if (!p->opt_graph) release_graph(p);
   } else if (k == "phase_timing") {
     p->opt_phase_timing = value != 0;
     if (!p->opt_phase_timing) release_phase_events(p);
@@ -1846,19 +1881,16 @@ int tdtm_stats_merge(int64_t n, float* mip, int32_t* idx, float* shi, float* slo
   return 0;
 }
 
-int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* euler, int O, int start,
-                    int count, float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo,
-                    void* stream) {
-  if (!p || !template_dft || !euler || !mip || !idx || !shi || !slo || !qhi || !qlo) return fail("null argument");
-  if (!p->image_set || !p->filters_set) return fail("tdtm_match_set_image / set_filters not called");
-  if (count < 1 || count > p->B) return fail("count must be in [1, max_orientation_batch]");
-  if (start < 0 || start + count > O) return fail("orientation range out of bounds");
-  if ((long)p->C * p->D * O > 2147483647L) return fail("search space exceeds int32 global index");
-  hipStream_t st = (hipStream_t)stream;
+}  // extern "C"
+
+// One orientation batch [start, start + count) enqueued on `st`.  With o0p set (graph
+// capture) the kernels take the first orientation from device memory instead of `start`.
+static int enqueue_step(tdtm_match_plan* p, const void* template_dft, const float* euler, int O, int start, int count,
+                        float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo, hipStream_t st,
+                        const int* o0p) {
   const int K = count * p->D * p->C;
-  if (K >= 65535) return fail("orientations x defocus x pixel sizes per step must be < 65535");
   hipEvent_t* ev = nullptr;
-  if (p->opt_phase_timing) {
+  if (p->opt_phase_timing && o0p == nullptr) {
     std::array<hipEvent_t, TDTM_NUM_PHASES + 1> e{};
     for (auto& x : e) TDTM_HIP(hipEventCreate(&x));
     p->phase_events.push_back(e);
@@ -1866,7 +1898,8 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
     TDTM_HIP(hipEventRecord(ev[0], st));
   }
   const long ns = (long)count * p->h * p->wr;
-  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S);
+  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S,
+                                             o0p);
   TDTM_LAUNCHED();
   // the projection kernels take their batch size from p->K; run them on exactly K
   const int Ksave = p->K;
@@ -1884,9 +1917,81 @@ int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* 
   TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, p->H, st);
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[3], st));
+  p->o0p_launch = o0p;
   TDTM_SWITCH_ROWS(p->W, launch_rowpass, p, K, count, O, start, mip, idx, shi, slo, qhi, qlo, st);
+  p->o0p_launch = nullptr;
   TDTM_LAUNCHED();
   if (ev) TDTM_HIP(hipEventRecord(ev[4], st));
+  return 0;
+}
+
+static int step_checks(tdtm_match_plan* p, const void* template_dft, const float* euler, int O, float* mip,
+                       int32_t* idx, float* shi, float* slo, float* qhi, float* qlo) {
+  if (!p || !template_dft || !euler || !mip || !idx || !shi || !slo || !qhi || !qlo) return fail("null argument");
+  if (!p->image_set || !p->filters_set) return fail("tdtm_match_set_image / set_filters not called");
+  if ((long)p->C * p->D * O > 2147483647L) return fail("search space exceeds int32 global index");
+  if ((long)p->B * p->D * p->C >= 65535) return fail("orientations x defocus x pixel sizes per step must be < 65535");
+  return 0;
+}
+
+// Capture one full-batch step (first orientation from d_o0, advanced by B at its end) for
+// these arguments, unless the kept executable already is for them.
+static int ensure_graph(tdtm_match_plan* p, const void* template_dft, const float* euler, int O, float* mip,
+                        int32_t* idx, float* shi, float* slo, float* qhi, float* qlo) {
+  const std::array<const void*, 8> key{template_dft, euler, mip, idx, shi, slo, qhi, qlo};
+  if (p->gexec && key == p->graph_key && O == p->graph_O) return 0;
+  release_graph(p);
+  if (!p->cap_stream) TDTM_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+  TDTM_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_step(p, template_dft, euler, O, 0, p->B, mip, idx, shi, slo, qhi, qlo, p->cap_stream, p->d_o0);
+  if (!rc) {
+    k_advance_start<<<1, 1, 0, p->cap_stream>>>(p->d_o0, p->B);
+    if (hipGetLastError() != hipSuccess) rc = fail("kernel launch during capture failed");
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  p->graph = g;
+  TDTM_HIP(hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0));
+  p->graph_key = key;
+  p->graph_O = O;
+  return 0;
+}
+
+extern "C" {
+
+int tdtm_match_step(tdtm_match_plan_t p, const void* template_dft, const float* euler, int O, int start,
+                    int count, float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo,
+                    void* stream) {
+  if (step_checks(p, template_dft, euler, O, mip, idx, shi, slo, qhi, qlo)) return 1;
+  if (count < 1 || count > p->B) return fail("count must be in [1, max_orientation_batch]");
+  if (start < 0 || start + count > O) return fail("orientation range out of bounds");
+  return enqueue_step(p, template_dft, euler, O, start, count, mip, idx, shi, slo, qhi, qlo, (hipStream_t)stream,
+                      nullptr);
+}
+
+int tdtm_match_run(tdtm_match_plan_t p, const void* template_dft, const float* euler, int O, int start, int end,
+                   float* mip, int32_t* idx, float* shi, float* slo, float* qhi, float* qlo, void* stream) {
+  if (step_checks(p, template_dft, euler, O, mip, idx, shi, slo, qhi, qlo)) return 1;
+  if (start < 0 || end > O || start > end) return fail("orientation range out of bounds");
+  hipStream_t st = (hipStream_t)stream;
+  const int nfull = (end - start) / p->B;
+  int o = start;
+  if (p->opt_graph && !p->opt_phase_timing && nfull > 0) {
+    if (ensure_graph(p, template_dft, euler, O, mip, idx, shi, slo, qhi, qlo)) return 1;
+    k_set_start<<<1, 1, 0, st>>>(p->d_o0, start);
+    TDTM_LAUNCHED();
+    for (int i = 0; i < nfull; ++i) TDTM_HIP(hipGraphLaunch(p->gexec, st));
+    o = start + nfull * p->B;
+  }
+  for (; o < end; o += p->B) {
+    const int count = std::min(p->B, end - o);
+    if (enqueue_step(p, template_dft, euler, O, o, count, mip, idx, shi, slo, qhi, qlo, st, nullptr)) return 1;
+  }
   return 0;
 }
 
@@ -1968,7 +2073,7 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
     const int K = count * p->D * p->C;
     const long ns = (long)count * p->h * p->wr;
     k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, offsets, start, count, euler_pose,
-                                               p->S);
+                                               p->S, nullptr);
     TDTM_LAUNCHED();
     const int Ksave = p->K;
     p->K = K;

@@ -32,6 +32,7 @@ SIGNATURES = {
     "tdtm_match_set_filters": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "tdtm_stats_init": (c_int, [c_int64] + [c_void_p] * 6 + [c_void_p]),
     "tdtm_match_step": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int] + [c_void_p] * 6 + [c_void_p]),
+    "tdtm_match_run": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int] + [c_void_p] * 6 + [c_void_p]),
     "tdtm_match_finalize": (c_int, [c_int64, c_int64] + [c_void_p] * 6 + [c_void_p, c_int, c_void_p, c_int]
                             + [c_void_p] * 7 + [c_void_p]),
     "tdtm_stats_merge": (c_int, [c_int64] + [c_void_p] * 12 + [c_void_p]),

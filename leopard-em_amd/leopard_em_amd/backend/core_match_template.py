@@ -167,8 +167,11 @@ class MatchPlan:
             _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums], self._stream()))
 
     def run(self, template_dft: torch.Tensor, euler_deg: torch.Tensor, start: int, end: int) -> None:
-        for i in range(start, end, self.B):
-            self.step(template_dft, euler_deg, i, min(self.B, end - i))
+        """Orientations [start, end) in batches of B (tdtm_match_run: full batches replayed
+        from a captured HIP graph unless the plan option "graph" is 0)."""
+        _lib.check(self.lib.tdtm_match_run(
+            self.handle, _lib.ptr(template_dft), _lib.ptr(euler_deg), euler_deg.shape[0], start, end,
+            _lib.ptr(self.mip), _lib.ptr(self.best_index), *[_lib.ptr(s) for s in self.sums], self._stream()))
 
     def merge_from(self, other: "MatchPlan") -> None:
         """self.stats <- merge(self.stats, other.stats); other may live on another device."""

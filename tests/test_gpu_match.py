@@ -332,3 +332,50 @@ def test_async_staging_refused_for_sub_wave_row_blocks(cuda_device):
         with pytest.raises(_lib.TdtmError, match="at least one wave"):
             plan.set_option("rowpass_staging", mode)
     plan.set_option("rowpass_staging", 0)
+
+
+def test_graph_replay_bit_identical_to_direct_steps(cuda_device):
+    """tdtm_match_run with graph replays (default) against the same batches launched as
+    plain steps: every statistics map bit-identical.  The graph is replayed across two
+    run() calls with eager work and a synchronisation between them, the last batch is
+    partial (run as a plain step), and a second statistics set forces a re-capture."""
+    from leopard_em_amd.backend import MatchPlan, raw_statistics
+
+    print("graph replay vs direct steps: 1024x1024 h64 D2 O14 B4", flush=True)
+    args = _inputs(1024, 1024, 64, 2, 1, 14, seed=41)
+    img_dft, tdft, ctf, wf, dfs, pix, eul = args
+    tdft_d, eul_d = tdft.to(cuda_device), eul.to(cuda_device)
+
+    def fresh(graph):
+        p = MatchPlan(1024, 1024, 64, 4, 2, 1, cuda_device)
+        p.set_option("graph", graph)
+        p.set_inputs(img_dft, ctf, wf)
+        p.reset_stats()
+        return p
+
+    direct = fresh(0)
+    for i in range(0, 14, 4):
+        direct.step(tdft_d, eul_d, i, min(4, 14 - i))
+    torch.cuda.synchronize()
+    ref = raw_statistics(direct)
+
+    g = fresh(1)
+    g.run(tdft_d, eul_d, 0, 8)           # two replays
+    junk = torch.randn(1 << 20, device=cuda_device).sum()  # eager work between replays
+    torch.cuda.synchronize()
+    g.run(tdft_d, eul_d, 8, 14)          # one replay + a partial batch of 2
+    torch.cuda.synchronize()
+    got = raw_statistics(g)
+    for k in ref:
+        assert torch.equal(ref[k], got[k]), k
+    # new statistics tensors: the kept graph no longer matches and is re-captured
+    g.mip = torch.empty_like(g.mip)
+    g.best_index = torch.empty_like(g.best_index)
+    g.sums = [torch.empty_like(s) for s in g.sums]
+    g.reset_stats()
+    g.run(tdft_d, eul_d, 0, 14)
+    torch.cuda.synchronize()
+    got2 = raw_statistics(g)
+    for k in ref:
+        assert torch.equal(ref[k], got2[k]), k
+    assert float(junk) == float(junk)
