@@ -315,6 +315,9 @@ def main():
                     help="run one device copy of this many MiB first (reference kernel for HBM byte counters)")
     ap.add_argument("--no-phase-timing", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time plain step launches instead of graph replays")
+    ap.add_argument("--fft-engine", type=int, default=0,
+                    help="column transforms: 0 FP32 VALU; matrix cores 1 fp16 split, 2 bf16 split, "
+                         "3 fp16, 4 bf16 (4096-row grids)")
     ap.add_argument("--zcut", type=float, default=7.5, help="z-score cut-off of the recall measurement")
     ap.add_argument("--no-prep-timing", action="store_true",
                     help="skip timing the input preparation kernels (synthetic inputs only)")
@@ -382,6 +385,8 @@ def main():
 
     plan = MatchPlan(H, W, h, B, D, 1, dev)
     plan.set_option("graph", 0 if args.no_graph else 1)
+    if args.fft_engine:
+        plan.set_option("fft_engine", args.fft_engine)
     plan.set_inputs(img_dft, ctf, wf)
     plan.reset_stats()
     stream = torch.cuda.current_stream(dev)

@@ -28,6 +28,7 @@
 #include "common.hpp"
 #include "fft.hpp"
 #include "fft4.hpp"
+#include "fft_mfma.hpp"
 #include "tdtm.h"
 
 namespace tdtm {
@@ -35,6 +36,7 @@ namespace tdtm {
 constexpr int V = 8;   // values per thread in every FFT
 constexpr int CW = 2;  // columns per column-pass block (tile width)
 constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
+static_assert(MF_CW == CW && MF_TR == TR, "the matrix-core column pass writes the same tiles");
 constexpr int TDTM_LDS_BYTES = 160 * 1024;  // LDS one workgroup may allocate on gfx950
 constexpr int REFINE_CHUNKS_MAX = 64;        // correlation chunks of a refinement row pass
 
@@ -1377,6 +1379,8 @@ struct tdtm_match_plan {
   int opt_rowpass_twiddles = 1; // 0: twiddles from the global table, 1: LDS copy (staging 0 / 3 only)
   int opt_rowpass_rows = 1;     // rows of a tile row per row-pass block: 4, 2 or 1 (staging 0 / 3)
   int opt_phase_timing = 0;     // record HIP events between the phases of every step
+  int opt_fft_engine = 0;       // column transforms: 0 FP32 VALU; MFMA (H = 4096) 1 fp16 split data,
+                                // 2 bf16 split data, 3 fp16 rounded once per stage, 4 bf16 rounded once
   // per-step events at the phase boundaries (project | rowfwd | colpass | rowpass), read and
   // released by tdtm_match_plan_phase_times
   std::vector<std::array<hipEvent_t, TDTM_NUM_PHASES + 1>> phase_events;
@@ -1596,8 +1600,40 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
                      p->h, p->G, p->Gs, K, kchunk, ystore, (const float2*)p->tw, p->Gb, 0);
 }
 
+// Matrix-core column pass (H = 4096, whole columns stored): persistent grid of as many
+// 1024-thread blocks as are resident, pairs (b, b + 8) over tile columns 2m / 2m + 1.
+template <class ET, bool SPLIT>
+static void launch_colpass_mfma(tdtm_match_plan* p, int K, hipStream_t st) {
+  const void* kern = (const void*)k_colpass_mfma<ET, SPLIT>;
+  if (p->occ_kern != kern) {
+    int n = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_colpass_mfma<ET, SPLIT>, MF_BLOCK, 0) != hipSuccess ||
+        n < 1)
+      n = 1;
+    p->occ_kern = kern;
+    p->occ_per_cu = n;
+  }
+  long nblk = ((long)p->occ_per_cu * p->num_cu) / 16 * 16;
+  if (nblk < 16) nblk = 16;
+  const long total = (long)((p->G + 1) / 2) * K;
+  const int persist = (int)((total + nblk / 2 - 1) / (nblk / 2));
+  k_colpass_mfma<ET, SPLIT><<<dim3((unsigned)nblk), MF_BLOCK, 0, st>>>(p->Rb, p->Xt, p->h, p->G, p->Gs, K, p->tw,
+                                                                      p->Gb, persist);
+}
+
 template <int HH>
 static void launch_colpass(tdtm_match_plan* p, int K, int ystore, hipStream_t st) {
+  if constexpr (HH == MF_N) {
+    if (p->opt_fft_engine != 0 && ystore == HH && p->h <= 512) {
+      switch (p->opt_fft_engine) {
+        case 1: launch_colpass_mfma<_Float16, true>(p, K, st); break;
+        case 2: launch_colpass_mfma<__bf16, true>(p, K, st); break;
+        case 3: launch_colpass_mfma<_Float16, false>(p, K, st); break;
+        default: launch_colpass_mfma<__bf16, false>(p, K, st); break;
+      }
+      return;
+    }
+  }
   constexpr int T = HH / V;
   const int nz = (p->h + T - 1) / T;
   if (nz <= 1) launch_colpass_n<HH, 1>(p, K, ystore, st);
@@ -1827,6 +1863,13 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     if (value && (p->opt_rowpass_staging == 1 || p->opt_rowpass_staging == 2))
       return fail("rowpass_twiddles = 1 needs rowpass_staging 0 or 3");
     p->opt_rowpass_twiddles = value;
+  } else if (k == "fft_engine") {
+    if (value < 0 || value > 4)
+      return fail("fft_engine must be 0 (FP32 VALU) or 1..4 (MFMA: fp16 / bf16 with split data, fp16 / bf16 "
+                  "rounded once per stage)");
+    if (value != 0 && p->H != MF_N) return fail("fft_engine 1..4 need a 4096-row search grid");
+    p->opt_fft_engine = value;
+    release_graph(p);
   } else if (k == "graph") {
     p->opt_graph = value != 0;
     if (!p->opt_graph) release_graph(p);

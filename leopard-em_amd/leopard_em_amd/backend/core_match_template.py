@@ -17,6 +17,10 @@ output dictionary.  Differences, all deliberate:
   work of a step is one fused launch sequence on the current stream.
 * with the environment variable ``TDTM_AUTOTUNE=1`` each device's plan picks its
   launch shapes with ``backend.autotune`` (cached per shape and device).
+* ``TDTM_FFT_ENGINE=1`` (fp16) or ``=2`` (bf16) runs the 4096-point column transforms of
+  4096-row search grids on the matrix cores (radix-16 DFT matrix products with hi / lo
+  split matrices and FP32 accumulation; results within 1e-3 of the FP64 reference
+  instead of the FP32 path's 1e-4); other grids keep the FP32 path.
 * image sizes that are not powers of two (e.g. 4092 x 5760 K3 micrographs) run in the
   next power-of-two grid with the image zero-padded at the bottom / right: every
   position whose template footprint lies inside the image (the valid region the peak
@@ -295,6 +299,9 @@ def core_match_template(
     for r, dev in enumerate(devs):
         lo, hi = r * share, min(O, (r + 1) * share)
         plan = MatchPlan(H, W, h, batch, D, C, dev)
+        engine = int(os.environ.get("TDTM_FFT_ENGINE", "0"))
+        if engine and plan.Hg == 4096:
+            plan.set_option("fft_engine", engine)
         with torch.cuda.device(dev):
             plan.set_inputs(image_dft, ctf_filters, whitening_filter_template)
             tdft = template_dft.to(dev, torch.complex64).contiguous()
