@@ -139,84 +139,145 @@ def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
     return n / dt, cores, n, dt
 
 
-def bench_refine(args) -> dict:
-    """Refinement throughput: ``--refine-particles`` 512 x 512 particle crops against a
-    256^3 template, a shared local grid of 17^3 orientation offsets (+/-2 deg at 0.25 deg)
-    x 13 defocus offsets (+/-300 A at 50 A) x 5 pixel-size offsets, batches of 32
-    offsets.  Inputs are device-resident random spectra (timing-equivalent); one warm-up
-    particle, then all particles timed between HIP events."""
+def _refine_stack_inputs(dev, n, box, h, seed=0):
+    """Config-3 style refinement stack: ``n`` box x box crops cut around embedded particles
+    of 8 synthetic micrographs (seeded sizes 2048..4092 x 2048..5760, CTF defocus spread
+    +/-3000 A, SNR 0.05), each crop with its own local grid around its true pose perturbed
+    by U(-2, 2) deg: orientation offsets +/-hw at 0.25 deg with hw in {1.5, 1.75, 2.0}
+    (13^3, 15^3 or 17^3 offsets), defocus offsets +/-300 A at 50 A (13) and 1..9 pixel-size
+    offsets 0.005 A apart."""
+    import numpy as np
     import torch
 
-    from leopard_em_amd import core_refine_template
+    from leopard_em_amd import synthetic
+
+    rng = np.random.default_rng(seed)
+    t = synthetic.make_blob_template(seed, h, max(100, int(5000 * (h / 512) ** 3)), 200.0 * h / 512)
+    vol = t.volume(device=dev)
+    from leopard_em_amd.utils import preprocessing as pp
+
+    tdft = pp.volume_to_rfft_fourier_slice(vol).to(torch.complex64).contiguous()
+    per = [n // 8 + (1 if i < n % 8 else 0) for i in range(8)]
+    crops, poses, truth, du, dv = [], [], [], [], []
+    for m in range(8):
+        if per[m] == 0:
+            continue
+        H = int(rng.integers(2048, 4093)) // 2 * 2
+        W = int(rng.integers(2048, 5761)) // 2 * 2
+        mg = synthetic.make_micrograph(t, seed + 31 * m, (H, W), per[m], 12200.0, 11800.0, 30.0, snr=0.05,
+                                       defocus_spread=3000.0, margin=box // 2 + 1, device=dev)
+        img = mg.image.to(dev)
+        for i in range(per[m]):
+            cy, cx = (int(round(v)) for v in mg.centers_yx[i])
+            crop = img[cy - box // 2:cy + box // 2, cx - box // 2:cx + box // 2]
+            crop = (crop - crop.mean()) / crop.std()
+            X = torch.fft.rfft2(crop)
+            X[0, 0] = 0
+            crops.append(X.to(torch.complex64))
+            truth.append(mg.eulers[i])
+            poses.append(mg.eulers[i] + rng.uniform(-2.0, 2.0, 3))
+            du.append(float(mg.defocus[i]) + 200.0)
+            dv.append(float(mg.defocus[i]) - 200.0)
+    stack = torch.stack(crops)
+    offs, dfos, pxos = [], [], []
+    for _ in range(n):
+        hw = float(rng.choice([1.5, 1.75, 2.0]))
+        a = torch.arange(-hw, hw + 1e-4, 0.25)
+        offs.append(torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1).reshape(-1, 3))
+        dfos.append(torch.arange(-300.0, 300.1, 50.0))
+        nc = int(rng.integers(1, 10))
+        pxos.append((torch.arange(nc, dtype=torch.float32) - (nc - 1) / 2) * 0.005)
+    return (stack, tdft, torch.tensor(np.array(poses), dtype=torch.float32), offs, dfos,
+            torch.tensor(du), torch.tensor(dv), torch.full((n,), 30.0), pxos, np.array(truth))
+
+
+def bench_refine(args) -> dict:
+    """Batched refinement throughput on the config-3 workload (``_refine_stack_inputs``):
+    2000 crops of 512^2 against a 256^3 template (valid region 257^2), each crop with its
+    own grid, batches of 32 offsets, one ``refine_template_ragged`` call (tdtm_refine_batch,
+    full batches replayed from HIP graphs).  ``--refine-particles`` crops are timed between
+    HIP events (default a 24-crop sample, extrapolated to the 2000-crop job); mean / std
+    maps are 0 / 1 (z = cc), inputs resident on the device."""
+    import numpy as np
+    import torch
+
+    from leopard_em_amd.backend import refine_template_ragged
+    sys.path.insert(0, REPO)
 
     dev = torch.device("cuda:0")
-    n, H, h = args.refine_particles, 512, 256
-    g = torch.Generator(device=dev).manual_seed(0)
-    x = torch.randn((n, H, H // 2 + 1), dtype=torch.complex64, device=dev, generator=g)
-    tdft = torch.randn((h, h, h // 2 + 1), dtype=torch.complex64, device=dev, generator=g)
-    eul = torch.rand((n, 3), device=dev, generator=g) * torch.tensor([360.0, 180.0, 360.0], device=dev)
-    a = torch.arange(-2.0, 2.0001, 0.25)
-    offs = torch.stack(torch.meshgrid(a, a, a, indexing="ij"), -1).reshape(-1, 3)
-    dfo = torch.arange(-300.0, 300.1, 50.0)
-    pxo = torch.linspace(-0.02, 0.02, 5)
-    crop = (H - h + 1, H - h + 1)
+    n, box, h = args.refine_particles, 512, 256
+    print(f"[bench] refinement stack: {n} crops of {box}^2 from 8 micrographs, template {h}^3",
+          file=sys.stderr, flush=True)
+    stack, tdft, poses, offs, dfos, du, dv, da, pxos, truth = _refine_stack_inputs(dev, n, box, h)
+    crop = (box - h + 1, box - h + 1)
     mean = torch.zeros((n,) + crop, device=dev)
     std = torch.ones((n,) + crop, device=dev)
-    filt = torch.rand((n, h, h // 2 + 1), device=dev, generator=g)
-    du = torch.full((n,), 12000.0)
-    dv = torch.full((n,), 11800.0)
-    da = torch.full((n,), 30.0)
+    filt = torch.ones((n, h, h // 2 + 1), device=dev)
     kw = dict(voltage=300.0, spherical_aberration=2.7, amplitude_contrast_ratio=0.07, phase_shift=0.0,
               pixel_size=1.0, ctf_B_factor=0.0)
 
     def run(sl):
-        return core_refine_template(x[sl], tdft, eul[sl], offs, dfo, du[sl], dv[sl], da[sl], pxo, mean[sl], std[sl],
-                                    kw, filt[sl], dev, batch_size=32)
+        return refine_template_ragged(stack[sl], tdft, poses[sl], offs[sl], dfos[sl], du[sl], dv[sl], da[sl],
+                                      pxos[sl], mean[sl], std[sl], kw, filt[sl], dev, batch_size=32)
 
-    run(slice(0, 1))
+    run(slice(0, 1))  # warm-up (plan, graph capture)
     torch.cuda.synchronize(dev)
-    stream = torch.cuda.current_stream(dev)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    run(slice(0, n))
-    e1.record(stream)
-    e1.synchronize()
-    sec = e0.elapsed_time(e1) / 1e3
-    ncorr = n * offs.shape[0] * dfo.numel() * pxo.numel()
-    print(f"[bench] refine {n} particles: {sec:.2f} s, {ncorr / sec:.0f} corr/s", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    res = run(slice(0, n))
+    torch.cuda.synchronize(dev)
+    sec = time.perf_counter() - t0
+    counts = [int(o.shape[0]) * int(d.numel()) * int(p.numel()) for o, d, p in zip(offs, dfos, pxos)]
+    ncorr = sum(counts)
+    rate = ncorr / sec
+    # angular error of the refined pose against the true pose (rotation angle of R_ref^T R_true)
+    from leopard_em_amd.synthetic import _rotmat_zyz_np
+
+    errs = []
+    for i in range(n):
+        r1 = _rotmat_zyz_np(*[float(v) for v in res["refined_euler_angles"][i]])
+        r0 = _rotmat_zyz_np(*[float(v) for v in truth[i]])
+        c = (np.trace(r1.T @ r0) - 1) / 2
+        errs.append(float(np.degrees(np.arccos(np.clip(c, -1, 1)))))
+    # the 2000-crop job: average correlations per crop of this generator x 2000
+    mean_corr = float(np.mean([((2 * hw / 0.25 + 1) ** 3) for hw in (1.5, 1.75, 2.0)]) * 13 * 5)
+    print(f"[bench] refine {n} crops: {sec:.2f} s, {rate:.0f} corr/s, median angular error "
+          f"{np.median(errs):.2f} deg", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        # the float32 CPU oracle (reference algorithm) on one particle with a 5^3 offset grid
-        sys.path.insert(0, REPO)
+        # the float32 CPU oracle (reference algorithm) on one crop with a 5^3 offset grid x 13 defocus
         from oracle import search as osr
 
         cores = os.cpu_count() or 1
         torch.set_num_threads(cores)
-        g3 = torch.tensor([-2.0, -1.0, 0.0, 1.0, 2.0])
+        g3 = torch.tensor([-0.5, -0.25, 0.0, 0.25, 0.5])
         offs3 = torch.stack(torch.meshgrid(g3, g3, g3, indexing="ij"), -1).reshape(-1, 3)
-        cargs = (x[:1].cpu(), tdft.cpu(), eul[:1].cpu(), offs3, dfo, du[:1], dv[:1], da[:1], pxo, mean[:1].cpu(),
+        dfo3 = torch.arange(-300.0, 300.1, 50.0)
+        pxo3 = torch.tensor([0.0])
+        cargs = (stack[:1].cpu(), tdft.cpu(), poses[:1], offs3, dfo3, du[:1], dv[:1], da[:1], pxo3, mean[:1].cpu(),
                  std[:1].cpu(), kw, filt[:1].cpu())
         t0 = time.perf_counter()
         osr.core_refine_template(*cargs, batch_size=32)
         dt = time.perf_counter() - t0
-        nc = offs3.shape[0] * dfo.numel() * pxo.numel()
+        nc = offs3.shape[0] * dfo3.numel() * pxo3.numel()
         cpu = {"value": nc / dt, "unit": "correlations/s", "cores": cores, "kind": "port",
-               "sample": f"1 particle x {offs3.shape[0]} offsets x {dfo.numel()} defocus x {pxo.numel()} pixel sizes "
-                         f"= {nc} correlations in {dt:.1f} s, torch CPU float32 oracle"}
+               "sample": f"1 crop x {offs3.shape[0]} offsets x {dfo3.numel()} defocus = {nc} correlations in "
+                         f"{dt:.1f} s, torch CPU float32 oracle"}
         print(f"[bench] CPU baseline: {nc / dt:.1f} corr/s", file=sys.stderr, flush=True)
     return {
-        "metric": "refinement correlations/s (512^2 crops x 256^3 template)",
-        "value": ncorr / sec,
+        "metric": "refinement correlations/s (512^2 crops x 256^3 template, per-crop grids)",
+        "value": rate,
         "unit": "correlations/s",
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "f32",
-        "data": "synthetic (random particle spectra, template spectrum and filters)",
-        "config": {"workload": f"core_refine_template, {n} particles of {H}x{H}, {h}^3 template, "
-                               f"{offs.shape[0]} orientation offsets x {dfo.numel()} defocus x {pxo.numel()} "
-                               "pixel sizes, batches of 32 offsets",
-                   "particles_per_s": n / sec},
+        "data": "synthetic (crops of seeded synthetic micrographs with embedded particles; mean 0 / std 1 maps)",
+        "config": {"workload": f"batched ragged refinement, {n} crops of {box}x{box} from 8 micrographs "
+                               f"(2048..4092 x 2048..5760), {h}^3 template, per-crop grids: 13^3 / 15^3 / 17^3 "
+                               "offsets at 0.25 deg x 13 defocus x 1..9 pixel sizes, batches of 32 offsets",
+                   "crops_timed": n, "correlations_timed": ncorr, "seconds": sec,
+                   "crops_per_s": n / sec,
+                   "job_2000_crops_s_extrapolated": 2000 * mean_corr / rate,
+                   "median_angular_error_deg": float(np.median(errs))},
         "cpu_baseline": cpu,
     }
 
@@ -319,11 +380,13 @@ def main():
                     help="column transforms: 0 FP32 VALU; matrix cores 1 fp16 split, 2 bf16 split, "
                          "3 fp16, 4 bf16 (4096-row grids)")
     ap.add_argument("--zcut", type=float, default=7.5, help="z-score cut-off of the recall measurement")
+    ap.add_argument("--no-variants", action="store_true", help="skip timing the matrix-core column-pass variants")
     ap.add_argument("--no-prep-timing", action="store_true",
                     help="skip timing the input preparation kernels (synthetic inputs only)")
     ap.add_argument("--workload", choices=("match", "refine"), default="match",
                     help="refine: particle refinement throughput instead of the search")
-    ap.add_argument("--refine-particles", type=int, default=3)
+    ap.add_argument("--refine-particles", type=int, default=24,
+                    help="crops timed by --workload refine (2000 = the whole config-3 job)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -441,6 +504,31 @@ def main():
     print(f"[bench] ms/step over {n_cmp} steps: direct launches {ms_direct:.3f}, graph replays {ms_graph:.3f}",
           file=sys.stderr, flush=True)
 
+    # the matrix-core column-pass variants on the same plan and orientations (4096-row grids):
+    # throughput of plain steps and the column pass's phase time
+    variants = None
+    if plan.Hg == 4096 and args.fft_engine == 0 and not args.no_variants:
+        variants = {}
+        names = {1: "MFMA fp16, split hi/lo data and matrices", 2: "MFMA bf16, split hi/lo data and matrices"}
+        for e, name in names.items():
+            plan.set_option("fft_engine", e)
+            plan.run(tdft, eul, 0, 2 * B)
+            e0.record(stream)
+            plan.run(tdft, eul, 0, n_cmp * B)
+            e1.record(stream)
+            e1.synchronize()
+            ms_v = e0.elapsed_time(e1) / n_cmp
+            plan.set_option("phase_timing", 1)
+            for i in range(min(n_cmp, 16)):
+                plan.step(tdft, eul, i * B, B)
+            ph, nph = plan.phase_times()
+            plan.set_option("phase_timing", 0)
+            variants[f"fft_engine={e}"] = {"name": name, "ms_per_step": ms_v, "value": B * D / (ms_v / 1e3),
+                                           "colpass_ms": ph["colpass"] / nph, "steps": n_cmp}
+            print(f"[bench] fft_engine {e} ({name}): {ms_v:.3f} ms/step, colpass {ph['colpass'] / nph:.3f} ms",
+                  file=sys.stderr, flush=True)
+        plan.set_option("fft_engine", 0)
+
     phases = None
     if not args.no_phase_timing:
         # HIP events between the phases of each step (plain launches)
@@ -531,6 +619,7 @@ def main():
             "dominant_kernel": dominant,
         },
         "recall": recall,
+        "variants": variants,
         "cpu_baseline": cpu,
         "preparation": prep,
     }

@@ -196,6 +196,35 @@ int tdtm_refine_particle(tdtm_match_plan_t plan, const void* template_dft, const
                          const float* projective_filter, const float* corr_mean, const float* corr_std,
                          float* out, void* stream);
 
+/*
+ * Refinement of a whole particle stack in one call: tdtm_refine_particle for every
+ * particle i, each with its own grid -- orientation offsets offsets[off_start[i] ..
+ * + off_count[i]) (float32 (., 3) on the device), defocus offsets defocus_offsets[d_start[i]
+ * .. + d_count[i]) and pixel-size offsets pixel_offsets[c_start[i] .. + c_count[i])
+ * (float32 on the device; d_count[i] <= the plan's num_defocus, c_count[i] <= its
+ * num_pixel) -- its own defocus_u / v / angle (host double arrays) and projective filter,
+ * mean and std maps.  Replaces the particle loop of core_refine_template
+ * (src/leopard_em/backend/core_refine_template.py:206-327 splits the stack over devices,
+ * :555-767 refines one particle) for shared grids and for uneven per-particle grids.
+ *   particle_dft        complex64, particle i at particle_dft + i * particle_stride
+ *                       (complex elements, >= grid H x (W/2+1)), resident on the device
+ *   euler_poses         float32 (n, 3);  projective_filters float32 (n, h, h/2+1)
+ *   corr_mean/corr_std  float32 (n, crop H-h+1, crop W-h+1);  out float32 (n, 14) as
+ *                       tdtm_refine_particle
+ *   off_start .. c_count host int arrays of n entries (starts index the device arrays)
+ * With the plan option "graph" (default 1) the full batches of offsets are replayed from
+ * HIP graphs captured once per (defocus count, pixel-size count): the first offset, the
+ * result slot and the particle index are read from device memory and advanced by the
+ * graph.  Results are identical to tdtm_refine_particle per particle.
+ */
+int tdtm_refine_batch(tdtm_match_plan_t plan, const void* template_dft, const void* particle_dft, int n_particles,
+                      int64_t particle_stride, const float* euler_poses, const float* offsets, const int* off_start,
+                      const int* off_count, const float* defocus_offsets, const int* d_start, const int* d_count,
+                      const float* pixel_offsets, const int* c_start, const int* c_count, const double* defocus_u,
+                      const double* defocus_v, const double* defocus_angle, const double* ctf_params,
+                      const float* projective_filters, const float* corr_mean, const float* corr_std, float* out,
+                      void* stream);
+
 /* ---------------------------------------------------------------- input preparation
  * Once-per-search inputs built on the device.  The reference builds them on the host
  * in MatchTemplateManager.make_backend_core_function_kwargs

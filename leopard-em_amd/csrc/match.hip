@@ -1044,6 +1044,20 @@ __global__ __launch_bounds__(RB*(WS / 2) / V, 4) void k_rowpass(const float2* __
 // Device-side first orientation of the next captured step.
 __global__ void k_set_start(int* o0p, int v) { *o0p = v; }
 __global__ void k_advance_start(int* o0p, int step) { *o0p += step; }
+// Batched refinement: state = {first offset of the next batch, its result slot, particle}.
+__global__ void k_refine_state(int* state, int o0, int slot, int part, const float* __restrict__ pose,
+                               float* __restrict__ pose_dst) {
+  state[0] = o0;
+  state[1] = slot;
+  state[2] = part;
+  pose_dst[0] = pose[0];
+  pose_dst[1] = pose[1];
+  pose_dst[2] = pose[2];
+}
+__global__ void k_refine_advance(int* state, int step) {
+  state[0] += step;
+  state[1] += 1;
+}
 
 __global__ void k_stats_init(long n, float* mip, int* idx, float* a, float* b, float* c, float* d) {
   const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1152,13 +1166,20 @@ __device__ __forceinline__ bool ref_better(float z, int pix, int k, float zb, in
 // z is evaluated once per pixel after the loop.  Block (tile row, chunk): the K
 // correlations are split into chunks of kchunk so that the grid fills the GPU even for
 // small crops (a 512 x 512 crop with a 256^3 template has 65 tile rows).
+// partp != nullptr (graph replays of a batched refinement): mean / stdv are the bases of
+// per-particle maps `plane` floats apart and *partp is the particle.
 template <int WS>
 __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2* __restrict__ Gb, int HH, int K,
                                                                    int kchunk, int crop_h, int crop_w,
                                                                    const float* __restrict__ mean,
-                                                                   const float* __restrict__ stdv,
+                                                                   const float* __restrict__ stdv, long plane,
+                                                                   const int* __restrict__ partp,
                                                                    const float2* __restrict__ tw,
                                                                    RefineBest* __restrict__ part) {
+  if (partp != nullptr) {
+    mean += (long)*partp * plane;
+    stdv += (long)*partp * plane;
+  }
   using C = RowCfg<WS>;
   constexpr int T = C::T;
   __shared__ __attribute__((aligned(16))) float lds[RowLds<WS, 3>::floats + RowTw<WS>::floats];
@@ -1227,8 +1248,11 @@ __global__ __launch_bounds__(TR*(WS / 2) / V) void k_rowpass_refine(const float2
   if (threadIdx.x == 0) part[(long)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
-// Reduce the row-pass blocks of one batch into its slot.
-__global__ void k_refine_reduce(const RefineBest* __restrict__ part, int n, RefineBest* __restrict__ slot) {
+// Reduce the row-pass blocks of one batch into slot bi of `slots` (slotp != nullptr:
+// slot *slotp, graph replays).
+__global__ void k_refine_reduce(const RefineBest* __restrict__ part, int n, RefineBest* __restrict__ slots, int bi,
+                                const int* __restrict__ slotp) {
+  RefineBest* slot = slots + (slotp != nullptr ? *slotp : bi);
   __shared__ RefineBest red[256];
   RefineBest b{-INFINITY, -INFINITY, 0x7fffffff, 0x7fffffff};
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -1411,6 +1435,17 @@ struct tdtm_match_plan {
   hipGraphExec_t gexec = nullptr;
   std::array<const void*, 8> graph_key{};
   int graph_O = -1;
+  // batched refinement: device state {offset start, slot, particle}, the current pose, and
+  // one captured full-batch graph per (defocus count, pixel count, arguments)
+  int* d_rstate = nullptr;
+  float* d_rpose = nullptr;
+  struct RefineGraph {
+    int D, C;
+    std::array<const void*, 5> key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<RefineGraph> rgraphs;
 };
 
 template <class T>
@@ -1437,9 +1472,20 @@ static void release_graph(tdtm_match_plan* p) {
   p->graph_O = -1;
 }
 
+static void release_refine_graphs(tdtm_match_plan* p) {
+  for (auto& g : p->rgraphs) {
+    (void)hipGraphExecDestroy(g.exec);
+    (void)hipGraphDestroy(g.graph);
+  }
+  p->rgraphs.clear();
+}
+
 static void plan_free(tdtm_match_plan* p) {
   release_phase_events(p);
   release_graph(p);
+  release_refine_graphs(p);
+  if (p->d_rstate) (void)hipFree(p->d_rstate);
+  if (p->d_rpose) (void)hipFree(p->d_rpose);
   if (p->cap_stream) (void)hipStreamDestroy(p->cap_stream);
   if (p->d_o0) (void)hipFree(p->d_o0);
   void* bufs[] = {p->tw, p->Xt, p->filt, p->S, p->Tb, p->P, p->part, p->ae, p->Rb, p->Gb, p->ctfbuf, p->rpart,
@@ -1697,10 +1743,10 @@ static void launch_rowpass_pow2(tdtm_match_plan* p, int K, int nb, int O, int o0
 
 template <int WS>
 static void launch_rowpass_refine(tdtm_match_plan* p, int K, int kchunk, int crop_h, int crop_w, const float* mean,
-                                  const float* sd, hipStream_t st) {
+                                  const float* sd, long plane, const int* partp, hipStream_t st) {
   const dim3 grid((crop_h + TR - 1) / TR, (K + kchunk - 1) / kchunk);
-  k_rowpass_refine<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, K, kchunk, crop_h, crop_w, mean, sd, p->tw,
-                                                           p->rpart);
+  k_rowpass_refine<WS><<<grid, TR * (WS / 2) / V, 0, st>>>(p->Gb, p->H, K, kchunk, crop_h, crop_w, mean, sd, plane,
+                                                           partp, p->tw, p->rpart);
 }
 
 #define TDTM_SWITCH_POW2(val, FN, ...)             \
@@ -1788,6 +1834,8 @@ int tdtm_match_plan_create(int H, int W, int h, int B, int D, int C, tdtm_match_
   rc |= rc ? 0 : dalloc(&p->ctfbuf, (size_t)C * D * h * p->wr, p->bytes);
   rc |= rc ? 0 : dalloc(&p->rpart, (size_t)H / TR * REFINE_CHUNKS_MAX, p->bytes);
   rc |= rc ? 0 : dalloc(&p->d_o0, 1, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->d_rstate, 3, p->bytes);
+  rc |= rc ? 0 : dalloc(&p->d_rpose, 3, p->bytes);
   if (rc) {
     plan_free(p);
     delete p;
@@ -1872,7 +1920,10 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     release_graph(p);
   } else if (k == "graph") {
     p->opt_graph = value != 0;
-    if (!p->opt_graph) release_graph(p);
+    if (!p->opt_graph) {
+      release_graph(p);
+      release_refine_graphs(p);
+    }
   } else if (k == "phase_timing") {
     p->opt_phase_timing = value != 0;
     if (!p->opt_phase_timing) release_phase_events(p);
@@ -2076,6 +2127,109 @@ int tdtm_match_finalize(int64_t n, int64_t total, const float* mip, const int32_
   return 0;
 }
 
+}  // extern "C"
+
+// Launches of one refinement batch of `count` offsets from `start` (K = count D C
+// correlations, the plan's D and C): slice (offsets composed with `pose`), projection,
+// padded forward FFT, column pass over the valid rows, row pass with the z epilogue against
+// mean / sd, and the reduction into result slot `bi`.  With `state` (graph capture) the
+// first offset, the slot and the particle index come from state[0..2] (mean / sd are then
+// the bases of per-particle maps `plane` floats apart).
+static int refine_batch(tdtm_match_plan* p, const void* template_dft, const float* offsets, int start, int count,
+                        const float* pose, const float* mean, const float* sd, long plane, int bi, const int* state,
+                        hipStream_t st) {
+  const int crop_h = p->Hn - p->h + 1, crop_w = p->Wn - p->h + 1;
+  const int ystore = ((crop_h + TR - 1) / TR) * TR;
+  const int nparts = (crop_h + TR - 1) / TR;
+  const int K = count * p->D * p->C;
+  const long ns = (long)count * p->h * p->wr;
+  k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, offsets, start, count, pose, p->S,
+                                             state);
+  TDTM_LAUNCHED();
+  const int Ksave = p->K;
+  p->K = K;
+  int rc = dispatch_proj(p, st);
+  p->K = Ksave;
+  if (rc) return rc;
+  TDTM_LAUNCHED();
+  k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->Hn, p->Wn, p->ae);
+  TDTM_LAUNCHED();
+  TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
+  TDTM_LAUNCHED();
+  TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, ystore, st);
+  TDTM_LAUNCHED();
+  // correlation chunks: about 8 blocks per CU over the tile rows of the crop
+  int nchunk = std::max(1, std::min({(8 * p->num_cu + nparts - 1) / nparts, K, REFINE_CHUNKS_MAX}));
+  const int kchunk = (K + nchunk - 1) / nchunk;
+  nchunk = (K + kchunk - 1) / kchunk;
+  TDTM_SWITCH_POW2(p->W, launch_rowpass_refine, p, K, kchunk, crop_h, crop_w, mean, sd, plane,
+                   state ? state + 2 : nullptr, st);
+  TDTM_LAUNCHED();
+  k_refine_reduce<<<1, 256, 0, st>>>(p->rpart, nparts * nchunk, p->rslots, bi, state ? state + 1 : nullptr);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+static int ensure_rslots(tdtm_match_plan* p, int nbatch) {
+  if (nbatch <= p->rslots_cap) return 0;
+  release_refine_graphs(p);  // captured reductions point at the old slots
+  if (p->rslots) TDTM_HIP(hipFree(p->rslots));
+  p->rslots = nullptr;
+  p->rslots_cap = 0;
+  if (dalloc(&p->rslots, (size_t)nbatch, p->bytes)) return 1;
+  p->rslots_cap = nbatch;
+  return 0;
+}
+
+// Captured full batch (B offsets) for the plan's current D and C, or the kept one.
+static int refine_graph(tdtm_match_plan* p, const void* template_dft, const float* offsets, const float* mean,
+                        const float* sd, long plane, hipGraphExec_t* out) {
+  const std::array<const void*, 5> key{template_dft, offsets, mean, sd, (const void*)p->rslots};
+  for (auto& g : p->rgraphs)
+    if (g.D == p->D && g.C == p->C && g.key == key) {
+      *out = g.exec;
+      return 0;
+    }
+  if (p->rgraphs.size() >= 16) release_refine_graphs(p);
+  if (!p->cap_stream) TDTM_HIP(hipStreamCreateWithFlags(&p->cap_stream, hipStreamNonBlocking));
+  TDTM_HIP(hipStreamBeginCapture(p->cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = refine_batch(p, template_dft, offsets, 0, p->B, p->d_rpose, mean, sd, plane, 0, p->d_rstate,
+                        p->cap_stream);
+  if (!rc) {
+    k_refine_advance<<<1, 1, 0, p->cap_stream>>>(p->d_rstate, p->B);
+    if (hipGetLastError() != hipSuccess) rc = fail("kernel launch during capture failed");
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(p->cap_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    return fail(std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  p->rgraphs.push_back({p->D, p->C, key, g, ex});
+  *out = ex;
+  return 0;
+}
+
+// CTF stack x projective filter of one particle into the plan's filters (D, C of the plan).
+static int refine_filters(tdtm_match_plan* p, double du, double dv, double da, const float* dfo, const float* pxo,
+                          const double* ctf_params, const float* projective_filter, hipStream_t st) {
+  if (launch_ctf_stack(p->h, p->D, p->C, du, dv, da, dfo, pxo, ctf_params, p->ctfbuf, st)) return 1;
+  const long plane = (long)p->h * p->wr;
+  const long tot = plane * p->D * p->C;
+  k_filters<<<(tot + 255) / 256, 256, 0, st>>>(p->ctfbuf, projective_filter, plane, tot, p->filt);
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+extern "C" {
+
 int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const void* particle_dft,
                          const float* euler_pose, const float* offsets, int n_off, const float* defocus_offsets,
                          const float* pixel_offsets, double defocus_u, double defocus_v, double defocus_angle,
@@ -2088,61 +2242,100 @@ int tdtm_refine_particle(tdtm_match_plan_t p, const void* template_dft, const vo
   if ((long)p->B * p->D * p->C >= 65535) return fail("batch x defocus x pixel sizes must be < 65535");
   hipStream_t st = (hipStream_t)stream;
   if (p->W > 4096) return fail("refinement crops are at most 4096 wide");
-  // valid region of the crop (the image inside a padded grid: tdtm_match_plan_set_image_size)
-  const int crop_h = p->Hn - p->h + 1, crop_w = p->Wn - p->h + 1;
+  const int crop_w = p->Wn - p->h + 1;
   const int nbatch = (n_off + p->B - 1) / p->B;
-  if (nbatch > p->rslots_cap) {
-    if (p->rslots) TDTM_HIP(hipFree(p->rslots));
-    p->rslots = nullptr;
-    p->rslots_cap = 0;
-    if (dalloc(&p->rslots, (size_t)nbatch, p->bytes)) return 1;
-    p->rslots_cap = nbatch;
-  }
+  if (ensure_rslots(p, nbatch)) return 1;
   // image tiles (match plan semantics: scale 1/(H W))
   if (tdtm_match_set_image(p, particle_dft, stream)) return 1;
-  // per-particle CTF stack and projective filter
-  if (launch_ctf_stack(p->h, p->D, p->C, defocus_u, defocus_v, defocus_angle, defocus_offsets, pixel_offsets,
-                       ctf_params, p->ctfbuf, st))
+  if (refine_filters(p, defocus_u, defocus_v, defocus_angle, defocus_offsets, pixel_offsets, ctf_params,
+                     projective_filter, st))
     return 1;
-  const long plane = (long)p->h * p->wr;
-  const long tot = plane * p->D * p->C;
-  k_filters<<<(tot + 255) / 256, 256, 0, st>>>(p->ctfbuf, projective_filter, plane, tot, p->filt);
-  TDTM_LAUNCHED();
-  const int ystore = ((crop_h + TR - 1) / TR) * TR;
-  const int nparts = (crop_h + TR - 1) / TR;
   for (int bi = 0; bi < nbatch; ++bi) {
     const int start = bi * p->B;
-    const int count = std::min(p->B, n_off - start);
-    const int K = count * p->D * p->C;
-    const long ns = (long)count * p->h * p->wr;
-    k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, offsets, start, count, euler_pose,
-                                               p->S, nullptr);
-    TDTM_LAUNCHED();
-    const int Ksave = p->K;
-    p->K = K;
-    int rc = dispatch_proj(p, st);
-    p->K = Ksave;
-    if (rc) return rc;
-    TDTM_LAUNCHED();
-    k_proj_norm<<<K, 256, 0, st>>>(p->part, p->h, p->Hn, p->Wn, p->ae);
-    TDTM_LAUNCHED();
-    TDTM_SWITCH_POW2(p->W, launch_rowfwd, p, K, st);
-    TDTM_LAUNCHED();
-    TDTM_SWITCH_POW2(p->H, launch_colpass, p, K, ystore, st);
-    TDTM_LAUNCHED();
-    // correlation chunks: about 8 blocks per CU over the tile rows of the crop
-    int nchunk = std::max(1, std::min({(8 * p->num_cu + nparts - 1) / nparts, K, REFINE_CHUNKS_MAX}));
-    const int kchunk = (K + nchunk - 1) / nchunk;
-    nchunk = (K + kchunk - 1) / kchunk;
-    TDTM_SWITCH_POW2(p->W, launch_rowpass_refine, p, K, kchunk, crop_h, crop_w, corr_mean, corr_std, st);
-    TDTM_LAUNCHED();
-    k_refine_reduce<<<1, 256, 0, st>>>(p->rpart, nparts * nchunk, p->rslots + bi);
-    TDTM_LAUNCHED();
+    if (refine_batch(p, template_dft, offsets, start, std::min(p->B, n_off - start), euler_pose, corr_mean, corr_std,
+                     0, bi, nullptr, st))
+      return 1;
   }
   k_refine_select<<<1, 64, 0, st>>>(p->rslots, nbatch, p->B, n_off, p->D, offsets, euler_pose, defocus_offsets,
                                      pixel_offsets, crop_w, out);
   TDTM_LAUNCHED();
   return 0;
+}
+
+int tdtm_refine_batch(tdtm_match_plan_t p, const void* template_dft, const void* particle_dft, int n_particles,
+                      int64_t particle_stride, const float* euler_poses, const float* offsets, const int* off_start,
+                      const int* off_count, const float* defocus_offsets, const int* d_start, const int* d_count,
+                      const float* pixel_offsets, const int* c_start, const int* c_count, const double* defocus_u,
+                      const double* defocus_v, const double* defocus_angle, const double* ctf_params,
+                      const float* projective_filters, const float* corr_mean, const float* corr_std, float* out,
+                      void* stream) {
+  if (!p || !template_dft || !particle_dft || !euler_poses || !offsets || !off_start || !off_count ||
+      !defocus_offsets || !d_start || !d_count || !pixel_offsets || !c_start || !c_count || !defocus_u ||
+      !defocus_v || !defocus_angle || !ctf_params || !projective_filters || !corr_mean || !corr_std || !out)
+    return fail("null argument");
+  if (n_particles < 1) return fail("no particles");
+  if (p->W > 4096) return fail("refinement crops are at most 4096 wide");
+  if (particle_stride < (int64_t)p->H * p->Wr) return fail("particle stride below the spectrum size");
+  const int Dmax = p->D, Cmax = p->C;
+  int maxbatch = 0;
+  for (int i = 0; i < n_particles; ++i) {
+    if (off_count[i] < 1 || off_start[i] < 0) return fail("particle " + std::to_string(i) + ": no orientation offsets");
+    if (d_count[i] < 1 || d_count[i] > Dmax || d_start[i] < 0)
+      return fail("particle " + std::to_string(i) + ": defocus count outside [1, plan num_defocus]");
+    if (c_count[i] < 1 || c_count[i] > Cmax || c_start[i] < 0)
+      return fail("particle " + std::to_string(i) + ": pixel-size count outside [1, plan num_pixel]");
+    maxbatch = std::max(maxbatch, (off_count[i] + p->B - 1) / p->B);
+  }
+  if ((long)p->B * Dmax * Cmax >= 65535) return fail("batch x defocus x pixel sizes must be < 65535");
+  hipStream_t st = (hipStream_t)stream;
+  if (ensure_rslots(p, maxbatch)) return 1;
+  const int crop_h = p->Hn - p->h + 1, crop_w = p->Wn - p->h + 1;
+  const long plane = (long)crop_h * crop_w;
+  const long fplane = (long)p->h * p->wr;
+  const float2* X = static_cast<const float2*>(particle_dft);
+  int rc = 0;
+  for (int i = 0; i < n_particles && !rc; ++i) {
+    p->D = d_count[i];
+    p->C = c_count[i];
+    const int n_off = off_count[i];
+    const float* offs = offsets + 3L * off_start[i];
+    const float* dfo = defocus_offsets + d_start[i];
+    const float* pxo = pixel_offsets + c_start[i];
+    const int nbatch = (n_off + p->B - 1) / p->B, nfull = n_off / p->B;
+    rc = tdtm_match_set_image(p, X + (long)i * particle_stride, stream);
+    if (!rc)
+      rc = refine_filters(p, defocus_u[i], defocus_v[i], defocus_angle[i], dfo, pxo, ctf_params,
+                          projective_filters + (long)i * fplane, st);
+    if (rc) break;
+    // device state: first offset, slot 0, particle i; pose copied to the plan's slot
+    k_refine_state<<<1, 1, 0, st>>>(p->d_rstate, off_start[i], 0, i, euler_poses + 3L * i, p->d_rpose);
+    if (hipGetLastError() != hipSuccess) {
+      rc = fail("kernel launch failed");
+      break;
+    }
+    int done = 0;
+    if (p->opt_graph && nfull > 0) {
+      hipGraphExec_t ex = nullptr;
+      rc = refine_graph(p, template_dft, offsets, corr_mean, corr_std, plane, &ex);
+      for (int b = 0; b < nfull && !rc; ++b) {
+        const hipError_t e = hipGraphLaunch(ex, st);
+        if (e != hipSuccess) rc = fail(std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+      }
+      done = nfull;
+    }
+    for (int bi = done; bi < nbatch && !rc; ++bi) {
+      const int start = bi * p->B;
+      rc = refine_batch(p, template_dft, offs, start, std::min(p->B, n_off - start), p->d_rpose,
+                        corr_mean + (long)i * plane, corr_std + (long)i * plane, 0, bi, nullptr, st);
+    }
+    if (rc) break;
+    k_refine_select<<<1, 64, 0, st>>>(p->rslots, nbatch, p->B, n_off, p->D, offs, p->d_rpose, dfo, pxo, crop_w,
+                                       out + 14L * i);
+    if (hipGetLastError() != hipSuccess) rc = fail("kernel launch failed");
+  }
+  p->D = Dmax;
+  p->C = Cmax;
+  return rc;
 }
 
 }  // extern "C"

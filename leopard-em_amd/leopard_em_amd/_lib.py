@@ -39,6 +39,7 @@ SIGNATURES = {
     "tdtm_refine_particle": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tdtm_refine_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64] + [c_void_p] * 18 + [c_void_p]),
     "tdtm_rfft2": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tdtm_volume_rfft": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "tdtm_irfft2_any_workspace_bytes": (c_int64, [c_int, c_int]),
