@@ -282,6 +282,36 @@ def bench_refine(args) -> dict:
     }
 
 
+def bench_proposal(args) -> dict:
+    """Learned orientation proposals (leopard_em_amd.proposal.run_experiment): the network
+    trained on the GPU by its own kernels on the plain search of 64 seeded 512^2
+    micrographs (64^3 template, 10 / 5 deg grid, 30 deg bins), the guided search on 16
+    held-out micrographs against the exhaustive search."""
+    import torch
+
+    from leopard_em_amd import proposal
+
+    dev = torch.device("cuda:0")
+    out = proposal.run_experiment(dev, n_train=64, n_test=16, n_particles=6, epochs=40,
+                                  operating_points=[(0.3, 0.9), (2.0, 0.9)],
+                                  log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
+    ex_rate = out["orientations"] * out["test_micrographs"] / out["exhaustive_s"]
+    return {
+        "metric": "proposal-guided search: (orientation x defocus)/s equivalent and recall of z > 7.5 peaks",
+        "value": out["orientations"] * out["test_micrographs"] / out["guided_s"],
+        "unit": "orientations/s (exhaustive-equivalent, 512^2 micrographs)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 network (f32 accumulation), f32 search",
+        "data": "synthetic (seeded micrographs, 6 embedded particles each, SNR 0.3)",
+        "config": {"workload": "learned orientation proposal, 64 training / 16 held-out micrographs 512^2, 64^3 "
+                               "template, 29,664 orientations, 552 bins at 30 deg, top 5% bins per 256^2 tile",
+                   "exhaustive_orientations_per_s": ex_rate},
+        "proposal": out,
+        "cpu_baseline": None,
+    }
+
+
 def kernel_sources_digest() -> str:
     """SHA-256 over the HIP sources of libtdtm (names and contents, sorted by name)."""
     import hashlib
@@ -383,8 +413,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true", help="skip timing the matrix-core column-pass variants")
     ap.add_argument("--no-prep-timing", action="store_true",
                     help="skip timing the input preparation kernels (synthetic inputs only)")
-    ap.add_argument("--workload", choices=("match", "refine"), default="match",
-                    help="refine: particle refinement throughput instead of the search")
+    ap.add_argument("--workload", choices=("match", "refine", "proposal"), default="match",
+                    help="refine: particle refinement throughput; proposal: learned orientation proposals")
     ap.add_argument("--refine-particles", type=int, default=24,
                     help="crops timed by --workload refine (2000 = the whole config-3 job)")
     args = ap.parse_args()
@@ -395,6 +425,9 @@ def main():
         sys.exit(2)
     if args.workload == "refine":
         print(json.dumps(bench_refine(args)), flush=True)
+        return
+    if args.workload == "proposal":
+        print(json.dumps(bench_proposal(args)), flush=True)
         return
 
     import torch

@@ -225,6 +225,32 @@ int tdtm_refine_batch(tdtm_match_plan_t plan, const void* template_dft, const vo
                       const float* projective_filters, const float* corr_mean, const float* corr_std, float* out,
                       void* stream);
 
+/* ------------------------------------------------------- orientation-proposal network
+ * No counterpart in the reference.  A small CNN that scores coarse orientation bins (and
+ * one "empty tile" class, the last) for 64 x 64 tiles (256 x 256 micrograph tiles averaged
+ * 4 x 4), trained on the orientation maps of the plain search; leopard_em_amd.proposal
+ * uses it to choose which orientations a tile's search visits.  Layers: four 3x3 conv +
+ * ReLU + 2x2 average pooling (16, 32, 64, 64 channels), fc 1024 -> 256 + ReLU, fc 256 ->
+ * num_classes.  Matrix products on bf16 MFMA with float32 accumulation; float32 master
+ * weights.  Parameter vector layout: per conv layer W (Cout, 3, 3, Cin) then b (Cout); W_fc1
+ * (256, 4, 4, 64) (inputs in (y, x, c) order) then b_fc1; W_fc2 (num_classes, 256) then
+ * b_fc2.  Tiles float32 (batch, 64, 64); logits / targets float32 (batch, num_classes),
+ * targets are probability distributions; all on the device.
+ */
+typedef struct tdtm_proposal_net* tdtm_proposal_net_t;
+int tdtm_proposal_create(int num_classes, int max_batch, tdtm_proposal_net_t* out);
+int tdtm_proposal_destroy(tdtm_proposal_net_t net);
+int64_t tdtm_proposal_num_params(tdtm_proposal_net_t net);
+/* Load parameters (float32, tdtm_proposal_num_params) and reset the optimiser state. */
+int tdtm_proposal_set_params(tdtm_proposal_net_t net, const float* params, void* stream);
+/* Copy out the parameters and / or the gradients of the last backward pass (NULL: skip). */
+int tdtm_proposal_get_params(tdtm_proposal_net_t net, float* params, float* grads, void* stream);
+int tdtm_proposal_forward(tdtm_proposal_net_t net, const float* tiles, int batch, float* logits, void* stream);
+/* Softmax cross-entropy of the last forward's logits against targets: per-sample loss
+ * (batch floats) and every parameter's gradient of the batch-mean loss. */
+int tdtm_proposal_backward(tdtm_proposal_net_t net, const float* targets, int batch, float* loss, void* stream);
+int tdtm_proposal_adam_step(tdtm_proposal_net_t net, float lr, float beta1, float beta2, float eps, void* stream);
+
 /* ---------------------------------------------------------------- input preparation
  * Once-per-search inputs built on the device.  The reference builds them on the host
  * in MatchTemplateManager.make_backend_core_function_kwargs

@@ -40,6 +40,14 @@ SIGNATURES = {
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double, c_void_p, c_void_p,
                                      c_void_p, c_void_p, c_void_p, c_void_p]),
     "tdtm_refine_batch": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int64] + [c_void_p] * 18 + [c_void_p]),
+    "tdtm_proposal_create": (c_int, [c_int, c_int, POINTER(c_void_p)]),
+    "tdtm_proposal_destroy": (c_int, [c_void_p]),
+    "tdtm_proposal_num_params": (c_int64, [c_void_p]),
+    "tdtm_proposal_set_params": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "tdtm_proposal_get_params": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tdtm_proposal_forward": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "tdtm_proposal_backward": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    "tdtm_proposal_adam_step": (c_int, [c_void_p, c_float, c_float, c_float, c_float, c_void_p]),
     "tdtm_rfft2": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "tdtm_volume_rfft": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
     "tdtm_irfft2_any_workspace_bytes": (c_int64, [c_int, c_int]),

@@ -12,7 +12,7 @@ REPO_DIR = os.path.dirname(PROJECT_DIR)
 CSRC = os.path.join(PROJECT_DIR, "csrc")
 INCLUDE = os.path.join(REPO_DIR, "include")
 
-SOURCES = ["match.hip", "prep.hip"]
+SOURCES = ["match.hip", "prep.hip", "proposal.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
