@@ -35,7 +35,7 @@ def algorithmic_bytes_per_correlation(H: int, W: int) -> int:
 
 
 def make_inputs(device, H, W, h, n_blobs, radius, n_particles, seed=0, return_micrograph=False):
-    """Config-2 style synthetic search inputs built on ``device``: Gaussian-blob template,
+    """Synthetic 2DTM search inputs built on ``device``: Gaussian-blob template,
     micrograph of embedded CTF-filtered projections at SNR 0.05 with per-particle defocus,
     whitened image spectrum, template rfft and the 13-plane CTF stack.  With
     ``return_micrograph`` the SyntheticMicrograph (true poses, centres, defocus) is
@@ -140,7 +140,7 @@ def cpu_baseline(img_dft, tdft, ctf, wf, eul, n_orient, batch):
 
 
 def _refine_stack_inputs(dev, n, box, h, seed=0):
-    """Config-3 style refinement stack: ``n`` box x box crops cut around embedded particles
+    """Refinement stack: ``n`` box x box crops cut around embedded particles
     of 8 synthetic micrographs (seeded sizes 2048..4092 x 2048..5760, CTF defocus spread
     +/-3000 A, SNR 0.05), each crop with its own local grid around its true pose perturbed
     by U(-2, 2) deg: orientation offsets +/-hw at 0.25 deg with hw in {1.5, 1.75, 2.0}
@@ -192,7 +192,7 @@ def _refine_stack_inputs(dev, n, box, h, seed=0):
 
 
 def bench_refine(args) -> dict:
-    """Batched refinement throughput on the config-3 workload (``_refine_stack_inputs``):
+    """Batched refinement throughput on crops of synthetic micrographs (``_refine_stack_inputs``):
     2000 crops of 512^2 against a 256^3 template (valid region 257^2), each crop with its
     own grid, batches of 32 offsets, one ``refine_template_ragged`` call (tdtm_refine_batch,
     full batches replayed from HIP graphs).  ``--refine-particles`` crops are timed between
@@ -312,17 +312,20 @@ def bench_proposal(args) -> dict:
     }
 
 
+SEARCH_SOURCES = ("common.hpp", "fft.hpp", "fft4.hpp", "fft_mfma.hpp", "match.hip")
+
+
 def kernel_sources_digest() -> str:
-    """SHA-256 over the HIP sources of libtdtm (names and contents, sorted by name)."""
+    """SHA-256 over the HIP sources of the search step (match.hip and the headers it
+    includes; names and contents in this order)."""
     import hashlib
 
     csrc = os.path.join(REPO, "leopard-em_amd", "csrc")
     h = hashlib.sha256()
-    for name in sorted(os.listdir(csrc)):
-        if name.endswith((".hip", ".hpp")):
-            h.update(name.encode())
-            with open(os.path.join(csrc, name), "rb") as f:
-                h.update(f.read())
+    for name in SEARCH_SOURCES:
+        h.update(name.encode())
+        with open(os.path.join(csrc, name), "rb") as f:
+            h.update(f.read())
     return h.hexdigest()
 
 
@@ -416,7 +419,7 @@ def main():
     ap.add_argument("--workload", choices=("match", "refine", "proposal"), default="match",
                     help="refine: particle refinement throughput; proposal: learned orientation proposals")
     ap.add_argument("--refine-particles", type=int, default=24,
-                    help="crops timed by --workload refine (2000 = the whole config-3 job)")
+                    help="crops timed by --workload refine (2000 = the whole 8-micrograph job)")
     args = ap.parse_args()
 
     if args.gpus != 1:

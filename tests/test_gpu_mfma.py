@@ -91,16 +91,16 @@ def test_mfma_column_pass_small(engine, cuda_device):
 
 @pytest.mark.timeout(1200)
 @pytest.mark.parametrize("engine", [1, 2], ids=["fp16-split", "bf16-split"])
-def test_mfma_column_pass_config2(engine, cuda_device):
+def test_mfma_column_pass_4096_t512(engine, cuda_device):
     """4096 x 4096 micrograph, 512^3 template, 13 defocus, 4 orientations (two true poses)."""
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import bench
     from leopard_em_amd.utils.search import get_uniform_euler_angles
 
-    from test_gpu_config2 import _margin64
+    from test_gpu_search_4096_t512 import _margin64
 
-    print(f"fft_engine {engine}: config 2 shape", flush=True)
+    print(f"fft_engine {engine}: 4096^2 x 512^3, 13 defocus", flush=True)
     img_dft, tdft, ctf, wf, dfs, vol, image, mg = bench.make_inputs(cuda_device, 4096, 4096, 512, 5000, 200.0, 200,
                                                                     return_micrograph=True)
     del vol, image

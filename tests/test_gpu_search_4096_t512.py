@@ -53,13 +53,13 @@ def _margin64(img_dft, tdft, filt, eul, batch):
 
 
 @pytest.mark.timeout(1200)
-def test_config2_parity_4096_t512_d13(cuda_device):
+def test_search_parity_4096_t512_d13(cuda_device):
     if REPO not in sys.path:
         sys.path.insert(0, REPO)
     import bench
     from leopard_em_amd.utils.search import get_uniform_euler_angles
 
-    print("config 2 parity: building 4096^2 x 512^3 inputs", flush=True)
+    print("4096^2 x 512^3 parity: building inputs", flush=True)
     img_dft, tdft, ctf, wf, dfs, vol, image, mg = bench.make_inputs(cuda_device, 4096, 4096, 512, 5000, 200.0, 200,
                                                                     return_micrograph=True)
     del vol, image
