@@ -39,24 +39,28 @@ constexpr int PN_HID = 256;
 // result.  One wave per 16 x 16 tile of C, four waves per block along n; lane l = j + 16 g
 // holds row j of the A tile and column j of the B tile for k = k0 + 8 g .. + 7 and rows
 // 4 g .. 4 g + 3 of column j of the result.
+// Split K (gridDim.z > 1): block z sums k in [z kchunk, (z + 1) kchunk) into its own
+// (M x N) slice C + z M N (ldc = N, beta 0); k_sum_splits adds the slices in order.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_gemm_bf16(int M, int N, int K, const __bf16* __restrict__ A, int lda,
                                                    const __bf16* __restrict__ B, int ldb, float* __restrict__ C,
-                                                   int ldc, int beta) {
+                                                   int ldc, int beta, int kchunk) {
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63, j = l & 15, g = l >> 4;
   const int m0 = blockIdx.y * 16, n0 = (blockIdx.x * 4 + wave) * 16;
   if (n0 >= N) return;  // whole wave leaves together
   const int row = m0 + j, col = n0 + j;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  C += (long)blockIdx.z * M * N;
   pn_f4 acc = {0.f, 0.f, 0.f, 0.f};
   const __bf16 zero = (__bf16)0.f;
-  for (int k0 = 0; k0 < K; k0 += 32) {
+  for (int k0 = kb; k0 < ke; k0 += 32) {
     pn_b8 a, b;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
       const int k = k0 + 8 * g + s;
-      const bool ka = row < M && k < K, kb = col < N && k < K;
-      a[s] = ka ? (TA ? A[(long)k * lda + row] : A[(long)row * lda + k]) : zero;
-      b[s] = kb ? (TB ? B[(long)col * ldb + k] : B[(long)k * ldb + col]) : zero;
+      const bool ia = row < M && k < ke, ib = col < N && k < ke;
+      a[s] = ia ? (TA ? A[(long)k * lda + row] : A[(long)row * lda + k]) : zero;
+      b[s] = ib ? (TB ? B[(long)col * ldb + k] : B[(long)k * ldb + col]) : zero;
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
   }
@@ -158,13 +162,61 @@ __global__ void k_to_bf16(const float* __restrict__ x, long n, __bf16* __restric
   if (e < n) y[e] = (__bf16)x[e];
 }
 
-// Column sums of a (rows x Cc) matrix (bias gradients), one thread per column, rows in order.
-__global__ void k_colsum(const float* __restrict__ G, long rows, int Cc, float* __restrict__ out) {
+// out[m ldc + n] (beta = 0: =, 1: +=) sum over z of the split-K slices part[z][m][n], in z order.
+__global__ void k_sum_splits(const float* __restrict__ part, int splits, int M, int N, float* __restrict__ C,
+                             int ldc, int beta) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)M * N) return;
+  float s = 0.f;
+  for (int z = 0; z < splits; ++z) s += part[(long)z * M * N + e];
+  float* c = C + (e / N) * ldc + e % N;
+  *c = beta ? *c + s : s;
+}
+
+// Column sums of a (rows x Cc) matrix (bias gradients), Cc dividing 256: block b takes rows
+// [b rchunk, (b + 1) rchunk); thread t adds column t % Cc over rows t / Cc, + 256 / Cc, ...
+// in order (consecutive threads read consecutive addresses), then the 256 / Cc phases are
+// added in a fixed tree; part[b][c] holds the block's sums.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ G, long rows, int Cc, long rchunk,
+                                                     float* __restrict__ part) {
+  __shared__ float red[256];
+  const int t = threadIdx.x, P = 256 / Cc, c = t % Cc, ph = t / Cc;
+  const long r0 = blockIdx.x * rchunk, r1 = min(rows, r0 + rchunk);
+  float s = 0.f;
+  for (long r = r0 + ph; r < r1; r += P) s += G[r * Cc + c];
+  red[t] = s;
+  __syncthreads();
+  for (int w = P / 2; w > 0; w >>= 1) {
+    if (ph < w) red[t] += red[t + w * Cc];
+    __syncthreads();
+  }
+  if (ph == 0) part[(long)blockIdx.x * Cc + c] = red[t];
+}
+
+// out[c] = sum over b of part[b][c], in b order.
+__global__ void k_colsum_final(const float* __restrict__ part, int nb, int Cc, float* __restrict__ out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= Cc) return;
   float s = 0.f;
-  for (long r = 0; r < rows; ++r) s += G[r * Cc + c];
+  for (int b = 0; b < nb; ++b) s += part[(long)b * Cc + c];
   out[c] = s;
+}
+
+// Column sums for any Cc: one block per column, thread t sums rows t, t + 256, ... in
+// order, then a fixed tree over the threads (used for the classifier's L columns).
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ G, long rows, int Cc,
+                                                float* __restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (long r = threadIdx.x; r < rows; r += 256) s += G[r * Cc + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
 }
 
 // Softmax cross-entropy against target distributions T (rows sum to 1): loss[n] and
@@ -248,6 +300,8 @@ struct tdtm_proposal_net {
   __bf16* ab = nullptr;    // bf16 copy of a GEMM's activation / gradient operand
   float* dbuf[3] = {};     // gradient scratch (largest activation)
   float* dcols = nullptr;
+  float* ws = nullptr;    // split-K slices of the weight-gradient products
+  float* csum = nullptr;  // per-block column sums (bias gradients)
   long cols_max = 0, act_max = 0;
   int steps = 0;
 };
@@ -262,7 +316,7 @@ int pn_alloc(void** p, size_t bytes) {
 
 void pn_free(tdtm_proposal_net* n) {
   void* bufs[] = {n->w, n->g, n->m, n->v, n->wb, n->h1, n->z, n->dz, n->loss, n->cols, n->ab, n->dcols,
-                  n->dbuf[0], n->dbuf[1], n->dbuf[2]};
+                  n->dbuf[0], n->dbuf[1], n->dbuf[2], n->ws, n->csum};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (float* a : n->act)
@@ -271,11 +325,65 @@ void pn_free(tdtm_proposal_net* n) {
 
 int edge(int l) { return PN_IN >> l; }  // spatial edge of the input of conv layer l
 
+// Split K for products with few output tiles and a long K (the weight gradients, K = batch
+// x pixels): aim at PN_SPLIT_TARGET blocks, at least PN_SPLIT_MIN_K per split.
+constexpr long PN_SPLIT_TARGET = 512, PN_SPLIT_MIN_K = 1024, PN_SPLIT_MAX = 128;
+constexpr long PN_COLSUM_BLOCKS = 128, PN_COLSUM_ROWS = 1024;
+
+// Upper bound on the splits of a product: non-decreasing in K.
+long split_bound(int M, int N, int K) {
+  const long base = (long)((N + 63) / 64) * ((M + 15) / 16);
+  const long s = std::min((PN_SPLIT_TARGET + base - 1) / base, std::max(1L, (long)K / PN_SPLIT_MIN_K));
+  return std::max(1L, std::min(s, PN_SPLIT_MAX));
+}
+
+// (splits, k per split) of a product, splits <= split_bound; splits = 1 runs the whole K
+// in one block column.
+void gemm_splits(int M, int N, int K, int* splits, int* kchunk) {
+  const long s = split_bound(M, N, K);
+  const long kc = ((K + s - 1) / s + 31) / 32 * 32;
+  *kchunk = (int)kc;
+  *splits = (int)((K + kc - 1) / kc);
+}
+
+// Floats of split-K slices any product of this shape with K' <= K needs.
+long splits_floats(int M, int N, int K) {
+  const long s = split_bound(M, N, K);
+  return s > 1 ? s * M * N : 0;
+}
+
+// C = / += A B; with ws (room for splits_floats(M, N, K) floats) the product is split
+// over K and the slices summed in order.
 template <bool TA, bool TB>
 int gemm(int M, int N, int K, const __bf16* A, int lda, const __bf16* B, int ldb, float* C, int ldc, int beta,
-         hipStream_t st) {
-  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 15) / 16));
-  k_gemm_bf16<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta);
+         hipStream_t st, float* ws = nullptr) {
+  int splits = 1, kchunk = K;
+  if (ws) gemm_splits(M, N, K, &splits, &kchunk);
+  if (splits == 1) kchunk = K;
+  const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 15) / 16), (unsigned)splits);
+  if (splits > 1) {
+    k_gemm_bf16<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, ws, N, 0, kchunk);
+    TDTM_LAUNCHED();
+    k_sum_splits<<<nblk((long)M * N), 256, 0, st>>>(ws, splits, M, N, C, ldc, beta);
+  } else {
+    k_gemm_bf16<TA, TB><<<grid, 256, 0, st>>>(M, N, K, A, lda, B, ldb, C, ldc, beta, kchunk);
+  }
+  TDTM_LAUNCHED();
+  return 0;
+}
+
+long colsum_blocks(long rows) { return std::min(PN_COLSUM_BLOCKS, (rows + PN_COLSUM_ROWS - 1) / PN_COLSUM_ROWS); }
+
+// out[c] = sum over rows of G[r][c] (fixed order: deterministic).
+int colsum(tdtm_proposal_net* n, const float* G, long rows, int Cc, float* out, hipStream_t st) {
+  if (Cc <= 256 && 256 % Cc == 0 && rows > PN_COLSUM_ROWS) {
+    const long nb = colsum_blocks(rows), rchunk = (rows + nb - 1) / nb;
+    k_colsum_part<<<(unsigned)nb, 256, 0, st>>>(G, rows, Cc, rchunk, n->csum);
+    TDTM_LAUNCHED();
+    k_colsum_final<<<nblk(Cc), 256, 0, st>>>(n->csum, (int)nb, Cc, out);
+  } else {
+    k_colsum<<<(unsigned)Cc, 256, 0, st>>>(G, rows, Cc, out);
+  }
   TDTM_LAUNCHED();
   return 0;
 }
@@ -323,10 +431,10 @@ int pn_backward(tdtm_proposal_net* n, int Nb, hipStream_t st) {
   if (to_bf16(n->dz, (long)Nb * n->L, n->ab, st)) return 1;
   __bf16* hb = reinterpret_cast<__bf16*>(n->dcols);  // bf16 copy of h1 (dcols is free here)
   if (to_bf16(n->h1, (long)Nb * PN_HID, hb, st)) return 1;
-  if (gemm<true, false>(n->L, PN_HID, Nb, n->ab, n->L, hb, PN_HID, n->g + n->woff[PN_LAYERS + 1], PN_HID, 0, st))
+  if (gemm<true, false>(n->L, PN_HID, Nb, n->ab, n->L, hb, PN_HID, n->g + n->woff[PN_LAYERS + 1], PN_HID, 0, st,
+                        n->ws))
     return 1;
-  k_colsum<<<nblk(n->L), 256, 0, st>>>(n->dz, Nb, n->L, n->g + n->boff[PN_LAYERS + 1]);
-  TDTM_LAUNCHED();
+  if (colsum(n, n->dz, Nb, n->L, n->g + n->boff[PN_LAYERS + 1], st)) return 1;
   if (gemm<false, false>(Nb, PN_HID, n->L, n->ab, n->L, n->wb + n->woff[PN_LAYERS + 1], PN_HID, dh1, PN_HID, 0, st))
     return 1;
   k_relu_grad<<<nblk((long)Nb * PN_HID), 256, 0, st>>>(dh1, n->h1, (long)Nb * PN_HID);
@@ -335,10 +443,10 @@ int pn_backward(tdtm_proposal_net* n, int Nb, hipStream_t st) {
   float* dF = n->dbuf[1];
   if (to_bf16(dh1, (long)Nb * PN_HID, n->ab, st)) return 1;
   if (to_bf16(n->act[2 * PN_LAYERS], (long)Nb * PN_FEAT, hb, st)) return 1;
-  if (gemm<true, false>(PN_HID, PN_FEAT, Nb, n->ab, PN_HID, hb, PN_FEAT, n->g + n->woff[PN_LAYERS], PN_FEAT, 0, st))
+  if (gemm<true, false>(PN_HID, PN_FEAT, Nb, n->ab, PN_HID, hb, PN_FEAT, n->g + n->woff[PN_LAYERS], PN_FEAT, 0, st,
+                        n->ws))
     return 1;
-  k_colsum<<<nblk(PN_HID), 256, 0, st>>>(dh1, Nb, PN_HID, n->g + n->boff[PN_LAYERS]);
-  TDTM_LAUNCHED();
+  if (colsum(n, dh1, Nb, PN_HID, n->g + n->boff[PN_LAYERS], st)) return 1;
   if (gemm<false, false>(Nb, PN_FEAT, PN_HID, n->ab, PN_HID, n->wb + n->woff[PN_LAYERS], PN_FEAT, dF, PN_FEAT, 0, st))
     return 1;
   // conv layers, last to first; dP = gradient of the pooled output of layer l
@@ -351,13 +459,13 @@ int pn_backward(tdtm_proposal_net* n, int Nb, hipStream_t st) {
     TDTM_LAUNCHED();
     k_relu_grad<<<nblk(rows * Co), 256, 0, st>>>(dY, n->act[2 * l + 1], rows * Co);
     TDTM_LAUNCHED();
-    k_colsum<<<nblk(Co), 256, 0, st>>>(dY, rows, Co, n->g + n->boff[l]);
-    TDTM_LAUNCHED();
+    if (colsum(n, dY, rows, Co, n->g + n->boff[l], st)) return 1;
     // dW (Co x 9 Ci) = dY^T cols; cols rebuilt from the layer input
     k_im2col<<<nblk(rows * 9 * Ci), 256, 0, st>>>(n->act[2 * l], Nb, E, E, Ci, n->cols);
     TDTM_LAUNCHED();
     if (to_bf16(dY, rows * Co, n->ab, st)) return 1;
-    if (gemm<true, false>(Co, 9 * Ci, (int)rows, n->ab, Co, n->cols, 9 * Ci, n->g + n->woff[l], 9 * Ci, 0, st))
+    if (gemm<true, false>(Co, 9 * Ci, (int)rows, n->ab, Co, n->cols, 9 * Ci, n->g + n->woff[l], 9 * Ci, 0, st,
+                          n->ws))
       return 1;
     if (l == 0) break;  // no gradient of the network input
     // dcols (rows x 9 Ci) = dY W; dX = col2im(dcols), the gradient of pooled output l - 1
@@ -429,6 +537,13 @@ int tdtm_proposal_create(int num_classes, int max_batch, tdtm_proposal_net_t* ou
   rc |= rc ? 0 : pn_alloc((void**)&n->ab, std::max(cmax, amax) * 2);
   rc |= rc ? 0 : pn_alloc((void**)&n->dcols, std::max(cmax, amax) * 4);
   for (int i = 0; i < 3 && !rc; ++i) rc |= pn_alloc((void**)&n->dbuf[i], amax * 4);
+  // split-K slices: the largest over the weight-gradient products at the largest batch
+  // (splits grow with K, so smaller batches need no more)
+  long wsmax = std::max(splits_floats(n->L, PN_HID, max_batch), splits_floats(PN_HID, PN_FEAT, max_batch));
+  for (int l = 0; l < PN_LAYERS; ++l)
+    wsmax = std::max(wsmax, splits_floats(PN_CH[l + 1], 9 * PN_CH[l], (int)(N * edge(l) * edge(l))));
+  rc |= rc ? 0 : pn_alloc((void**)&n->ws, std::max(1L, wsmax) * 4);
+  rc |= rc ? 0 : pn_alloc((void**)&n->csum, PN_COLSUM_BLOCKS * 256 * 4);
   if (rc) {
     pn_free(n);
     delete n;
