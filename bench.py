@@ -284,16 +284,18 @@ def bench_refine(args) -> dict:
 
 def bench_proposal(args) -> dict:
     """Learned orientation proposals (leopard_em_amd.proposal.run_experiment): the network
-    trained on the GPU by its own kernels on the plain search of 64 seeded 512^2
-    micrographs (64^3 template, 10 / 5 deg grid, 30 deg bins), the guided search on 16
-    held-out micrographs against the exhaustive search."""
+    trained on the GPU by its own kernels on the plain search of ``--proposal-train``
+    seeded 512^2 micrographs (64^3 template, 10 / 5 deg grid, ``--proposal-bin-deg`` bins,
+    ``--proposal-tile`` tiles), the guided search on ``--proposal-test`` held-out
+    micrographs against the exhaustive search."""
     import torch
 
     from leopard_em_amd import proposal
 
     dev = torch.device("cuda:0")
-    out = proposal.run_experiment(dev, n_train=64, n_test=16, n_particles=6, epochs=40,
-                                  operating_points=[(0.3, 0.9), (2.0, 0.9)],
+    out = proposal.run_experiment(dev, n_train=args.proposal_train, n_test=args.proposal_test,
+                                  n_particles=args.proposal_particles, epochs=40, tile=args.proposal_tile,
+                                  bin_deg=args.proposal_bin_deg, operating_points=[(0.3, 0.9), (2.0, 0.9)],
                                   log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
     ex_rate = out["orientations"] * out["test_micrographs"] / out["exhaustive_s"]
     return {
@@ -303,9 +305,11 @@ def bench_proposal(args) -> dict:
         "n_gpus": 1,
         "higher_is_better": True,
         "dtype": "bf16 network (f32 accumulation), f32 search",
-        "data": "synthetic (seeded micrographs, 6 embedded particles each, SNR 0.3)",
-        "config": {"workload": "learned orientation proposal, 64 training / 16 held-out micrographs 512^2, 64^3 "
-                               "template, 29,664 orientations, 552 bins at 30 deg, top 5% bins per 256^2 tile",
+        "data": f"synthetic (seeded micrographs, {args.proposal_particles} embedded particles each, SNR 0.3)",
+        "config": {"workload": f"learned orientation proposal, {out['train_micrographs']} training / "
+                               f"{out['test_micrographs']} held-out micrographs 512^2, 64^3 template, "
+                               f"{out['orientations']:,} orientations, {out['bins']} bins at {out['bin_deg']:g} deg, "
+                               f"top 5% bins per {out['tile']}^2 tile",
                    "exhaustive_orientations_per_s": ex_rate},
         "proposal": out,
         "cpu_baseline": None,
@@ -420,6 +424,11 @@ def main():
                     help="refine: particle refinement throughput; proposal: learned orientation proposals")
     ap.add_argument("--refine-particles", type=int, default=24,
                     help="crops timed by --workload refine (2000 = the whole 8-micrograph job)")
+    ap.add_argument("--proposal-train", type=int, default=64, help="training micrographs of --workload proposal")
+    ap.add_argument("--proposal-test", type=int, default=16, help="held-out micrographs of --workload proposal")
+    ap.add_argument("--proposal-particles", type=int, default=6, help="particles per proposal micrograph")
+    ap.add_argument("--proposal-tile", type=int, default=256, help="tile edge (multiple of 64) of the proposals")
+    ap.add_argument("--proposal-bin-deg", type=float, default=30.0, help="coarse orientation bin step (deg)")
     args = ap.parse_args()
 
     if args.gpus != 1:

@@ -227,8 +227,8 @@ int tdtm_refine_batch(tdtm_match_plan_t plan, const void* template_dft, const vo
 
 /* ------------------------------------------------------- orientation-proposal network
  * No counterpart in the reference.  A small CNN that scores coarse orientation bins (and
- * one "empty tile" class, the last) for 64 x 64 tiles (256 x 256 micrograph tiles averaged
- * 4 x 4), trained on the orientation maps of the plain search; leopard_em_amd.proposal
+ * one "empty tile" class, the last) for 64 x 64 tiles (micrograph tiles of 64 s x 64 s pixels
+ * averaged s x s), trained on the orientation maps of the plain search; leopard_em_amd.proposal
  * uses it to choose which orientations a tile's search visits.  Layers: four 3x3 conv +
  * ReLU + 2x2 average pooling (16, 32, 64, 64 channels), fc 1024 -> 256 + ReLU, fc 256 ->
  * num_classes.  Matrix products on bf16 MFMA with float32 accumulation; float32 master

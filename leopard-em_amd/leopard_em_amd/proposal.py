@@ -1,8 +1,8 @@
 """Learned orientation proposals for the 2DTM search (no counterpart in the reference).
 
 A small CNN (``tdtm_proposal_*`` in ``csrc/proposal.hip``: four 3x3 conv layers on 64 x 64
-tiles, an MLP head) scores coarse orientation bins for each 256 x 256 tile of a
-micrograph, plus one "empty tile" class.  It is trained on the best-orientation maps the
+inputs, an MLP head) scores coarse orientation bins for each tile of a micrograph (256 x
+256 by default, averaged to 64 x 64), plus one "empty tile" class.  It is trained on the best-orientation maps the
 plain search produces on seeded synthetic micrographs.  The proposal-guided search then
 visits, per tile, only the fine orientations inside the tile's top-k bins (plus a fixed
 random subset used for the per-pixel mean / std), falls back to the exhaustive search for
@@ -29,8 +29,8 @@ import torch
 from . import _lib
 from .utils.search import get_uniform_euler_angles
 
-TILE = 256        # micrograph tile edge seen by the network
-DOWN = 4          # averaging factor: network input 64 x 64
+TILE = 256        # default micrograph tile edge seen by the network
+NET_IN = 64       # network input edge: a tile is averaged TILE / NET_IN x TILE / NET_IN
 CHANNELS = (1, 16, 32, 64, 64)
 HIDDEN = 256
 FEATURES = 4 * 4 * 64
@@ -189,12 +189,16 @@ def tile_grid(H: int, W: int, h: int, tile: int = TILE):
     return [(y, x, oy, ox) for y, oy in zip(ys, oys) for x, ox in zip(xs, oxs)]
 
 
-def tile_inputs(image: torch.Tensor, tiles) -> torch.Tensor:
-    """(n_tiles, 64, 64) network inputs: each tile crop averaged 4 x 4, zero mean, unit std."""
+def tile_inputs(image: torch.Tensor, tiles, tile: int = TILE) -> torch.Tensor:
+    """(n_tiles, 64, 64) network inputs: each tile x tile crop averaged over
+    (tile / 64)^2 pixels, zero mean, unit std."""
+    if tile % NET_IN:
+        raise ValueError(f"tile edge {tile} is not a multiple of {NET_IN}")
     out = []
     for y0, x0, _, _ in tiles:
-        c = image[y0:y0 + TILE, x0:x0 + TILE].float()
-        c = torch.nn.functional.avg_pool2d(c[None, None], DOWN)[0, 0]
+        c = image[y0:y0 + tile, x0:x0 + tile].float()
+        if tile > NET_IN:
+            c = torch.nn.functional.avg_pool2d(c[None, None], tile // NET_IN)[0, 0]
         out.append((c - c.mean()) / (c.std() + 1e-6))
     return torch.stack(out)
 
@@ -210,12 +214,12 @@ def peaks_of(z: torch.Tensor, zcut: float, radius: float = 5.0) -> np.ndarray:
 
 
 def training_examples(image: torch.Tensor, best_euler: torch.Tensor, z: torch.Tensor, h: int, bins: OrientationBins,
-                      zcut: float):
+                      zcut: float, tile: int = TILE):
     """Network inputs and targets of one searched micrograph: per tile, uniform over the
     bins of the best orientations at its z > zcut peaks, or the empty class."""
     H, W = image.shape
-    tiles = tile_grid(H, W, h)
-    x = tile_inputs(image, tiles)
+    tiles = tile_grid(H, W, h, tile)
+    x = tile_inputs(image, tiles, tile)
     pk = peaks_of(z, zcut)
     t = torch.zeros((len(tiles), bins.num_classes))
     for i, (_, _, (oy0, oy1), (ox0, ox1)) in enumerate(tiles):
@@ -230,8 +234,8 @@ def training_examples(image: torch.Tensor, best_euler: torch.Tensor, z: torch.Te
 
 
 # ---------------------------------------------------------------------- guided search
-def _tile_spectrum(real_image: torch.Tensor, y0: int, x0: int) -> torch.Tensor:
-    c = real_image[y0:y0 + TILE, x0:x0 + TILE].float().contiguous()
+def _tile_spectrum(real_image: torch.Tensor, y0: int, x0: int, tile: int) -> torch.Tensor:
+    c = real_image[y0:y0 + tile, x0:x0 + tile].float().contiguous()
     X = torch.fft.rfft2(c)
     X[0, 0] = 0
     return X.to(torch.complex64)
@@ -240,7 +244,7 @@ def _tile_spectrum(real_image: torch.Tensor, y0: int, x0: int) -> torch.Tensor:
 def guided_search(plan_cache: dict, net: ProposalNet, bins: OrientationBins, real_image: torch.Tensor,
                   tdft: torch.Tensor, ctf: torch.Tensor, wf: torch.Tensor, euler: torch.Tensor,
                   fine_bins: np.ndarray, stats_idx: torch.Tensor, h: int, top_k: int, conf: float,
-                  empty_conf: float, batch: int = 32) -> dict:
+                  empty_conf: float, batch: int = 32, tile: int = TILE) -> dict:
     """Proposal-guided search of one (whitened, real-space) micrograph; ``euler`` the fine
     grid on the device, ``fine_bins`` its bins, ``stats_idx`` the orientation subset every
     searched tile runs for its per-pixel mean / std.  Returns the z map of the valid
@@ -250,15 +254,15 @@ def guided_search(plan_cache: dict, net: ProposalNet, bins: OrientationBins, rea
 
     dev = euler.device
     H, W = real_image.shape
-    tiles = tile_grid(H, W, h)
-    x = tile_inputs(real_image, tiles)
+    tiles = tile_grid(H, W, h, tile)
+    x = tile_inputs(real_image, tiles, tile)
     prob = torch.softmax(net.forward(x).float(), dim=1).cpu()
     z = torch.full((H - h + 1, W - h + 1), -float("inf"))
     O = euler.shape[0]
     visited, decisions = 0, []
-    key = (TILE, h, batch)
+    key = (tile, h, batch)
     if key not in plan_cache:
-        plan_cache[key] = MatchPlan(TILE, TILE, h, batch, 1, 1, dev)
+        plan_cache[key] = MatchPlan(tile, tile, h, batch, 1, 1, dev)
         # every tile searches its own orientation subset (a new Euler array each time):
         # plain launches, no graph to re-capture per call
         plan_cache[key].set_option("graph", 0)
@@ -277,7 +281,7 @@ def guided_search(plan_cache: dict, net: ProposalNet, bins: OrientationBins, rea
             sel = np.nonzero(np.isin(fine_bins, top))[0]
             idx = torch.unique(torch.cat([torch.from_numpy(sel).to(dev), stats_idx]))
             decisions.append("proposal")
-        plan.set_inputs(_tile_spectrum(real_image, y0, x0), ctf, wf)
+        plan.set_inputs(_tile_spectrum(real_image, y0, x0, tile), ctf, wf)
         # per-pixel mean / std from the statistics subset
         es = euler[stats_idx].contiguous()
         plan.reset_stats()
@@ -326,7 +330,7 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
                    n_particles: int = 16, snr: float = 0.3, theta_step: float = 10.0, psi_step: float = 5.0,
                    bin_deg: float = 30.0, zcut: float = 7.5, epochs: int = 40, top_frac: float = 0.05,
                    conf: float = 0.02, empty_conf: float = 0.9, stats_frac: float = 0.05, batch: int = 32,
-                   seed: int = 0, log=None, operating_points=None) -> dict:
+                   seed: int = 0, log=None, operating_points=None, tile: int = TILE, lr: float = 1e-3) -> dict:
     """Train the proposal network on the plain search of ``n_train`` seeded micrographs and
     measure the guided search on ``n_test`` held-out ones against the exhaustive search:
     recall of its z > zcut peaks, orientations visited and wall time; beside it the same
@@ -375,7 +379,7 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
     for s in range(n_train):
         img_dft, wf, real, _ = _micrograph_inputs(t, 1000 + s, size, n_particles, snr, dev)
         z, be = exhaustive(img_dft, wf)
-        x, tt = training_examples(real, be, z, h, bins, zcut)
+        x, tt = training_examples(real, be, z, h, bins, zcut, tile)
         xs.append(x)
         ts.append(tt)
     torch.cuda.synchronize(dev)
@@ -384,7 +388,7 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
     say(f"{X.shape[0]} training tiles ({int((T[:, bins.empty] == 1).sum())} empty) in {gen_s:.1f} s")
     net = ProposalNet(bins.num_classes, 64, dev, seed=seed)
     t0 = time.perf_counter()
-    hist = net.train(X, T, epochs, lr=1e-3, batch=32, seed=seed)
+    hist = net.train(X, T, epochs, lr=lr, batch=32, seed=seed)
     torch.cuda.synchronize(dev)
     train_s = time.perf_counter() - t0
     say(f"training: loss {hist[0]:.3f} -> {hist[-1]:.3f} in {train_s:.1f} s")
@@ -404,6 +408,7 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
     res = {"net": [], "random": [], "exhaustive_s": 0.0, "guided_s": 0.0, "random_s": 0.0}
     counts = {"net": [0, 0], "random": [0, 0]}
     decisions = {"empty": 0, "exhaustive": 0, "proposal": 0}
+    hits = {"net": 0, "tiles": 0, "empty_right": 0, "empty_tiles": 0, "occupied_called_empty": 0}
     ops = [tuple(o) for o in (operating_points or [])]
     op_res = {o: {"recall": [], "visited": 0, "total": 0, "s": 0.0, "decisions": {"empty": 0, "exhaustive": 0,
                                                                                 "proposal": 0}} for o in ops}
@@ -411,14 +416,27 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
         img_dft, wf, real, _ = _micrograph_inputs(t, 5000 + s, size, n_particles, snr, dev)
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        z_ref, _ = exhaustive(img_dft, wf)
+        z_ref, be_ref = exhaustive(img_dft, wf)
         torch.cuda.synchronize(dev)
         res["exhaustive_s"] += time.perf_counter() - t0
         ref = peaks_of(z_ref, zcut)
+        # held-out tiles: is a bin of the exhaustive search's peaks among the network's top k?
+        xt, tt = training_examples(real, be_ref, z_ref, h, bins, zcut, tile)
+        occ = tt[:, bins.empty] < 1
+        if bool(occ.any()):
+            pr = torch.softmax(net.forward(xt[occ]).float(), dim=1).cpu()[:, :bins.num_bins]
+            top = torch.topk(pr, min(top_k, bins.num_bins), dim=1).indices
+            hits["net"] += int((torch.gather(tt[occ][:, :bins.num_bins], 1, top) > 0).any(1).sum())
+            hits["tiles"] += int(occ.sum())
+        hits["empty_right"] += int((torch.softmax(net.forward(xt[~occ]).float(), dim=1).cpu()[:, bins.empty]
+                                    > empty_conf).sum()) if bool((~occ).any()) else 0
+        hits["empty_tiles"] += int((~occ).sum())
+        hits["occupied_called_empty"] += int((torch.softmax(net.forward(xt[occ]).float(), dim=1).cpu()[:, bins.empty]
+                                              > empty_conf).sum()) if bool(occ.any()) else 0
         for name, model in (("net", net), ("random", RandomNet(seed + s))):
             t0 = time.perf_counter()
             g = guided_search(plans, model, bins, real, tdft, ctf, wf, euler, fine_bins, stats_idx, h, top_k,
-                              conf if name == "net" else 0.0, empty_conf if name == "net" else 2.0, batch)
+                              conf if name == "net" else 0.0, empty_conf if name == "net" else 2.0, batch, tile)
             torch.cuda.synchronize(dev)
             res["guided_s" if name == "net" else "random_s"] += time.perf_counter() - t0
             got = peaks_of(g["z"], zcut)
@@ -433,7 +451,7 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
         for o in ops:
             t0 = time.perf_counter()
             g = guided_search(plans, net, bins, real, tdft, ctf, wf, euler, fine_bins, stats_idx, h, top_k, o[0], o[1],
-                              batch)
+                              batch, tile)
             torch.cuda.synchronize(dev)
             r = op_res[o]
             r["s"] += time.perf_counter() - t0
@@ -459,6 +477,12 @@ def run_experiment(dev: torch.device, n_train: int = 64, n_test: int = 16, size:
         "visited_fraction_net": counts["net"][0] / max(1, counts["net"][1]),
         "visited_fraction_random": counts["random"][0] / max(1, counts["random"][1]),
         "tile_decisions": decisions,
+        "tile": tile,
+        "held_out_top_k_hit_rate": hits["net"] / max(1, hits["tiles"]),
+        "random_top_k_hit_rate": top_k / bins.num_bins,
+        "held_out_occupied_tiles": hits["tiles"],
+        "held_out_empty_tiles_called_empty": hits["empty_right"] / max(1, hits["empty_tiles"]),
+        "held_out_occupied_tiles_called_empty": hits["occupied_called_empty"] / max(1, hits["tiles"]),
         "exhaustive_s": res["exhaustive_s"], "guided_s": res["guided_s"], "random_s": res["random_s"],
         "speedup_vs_exhaustive": res["exhaustive_s"] / max(1e-9, res["guided_s"]),
         "operating_points": [
