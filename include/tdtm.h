@@ -64,6 +64,16 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
 int tdtm_match_plan_set_image_size(tdtm_match_plan_t plan, int image_h, int image_w);
 
 /*
+ * Anisotropic magnification (host doubles {a00, a01, a10, a11}, NULL = none): every central
+ * slice is resampled bilinearly at A k before the filters (torch_fourier_slice's
+ * transform_slice_2d at src/leopard_em/backend/cross_correlation.py:81-88, :231-238), and the
+ * per-particle CTF of refinement is evaluated at A k (the mag_matrix entry of ctf_kwargs,
+ * utils/ctf_utils.py:66-100).  A acts on (x, y) frequency column vectors; slice points whose
+ * transformed x is negative take the conjugate of the sample at -A k.  Default: none.
+ */
+int tdtm_match_plan_set_magnification(tdtm_match_plan_t plan, const double* mag_matrix);
+
+/*
  * Launch-shape options of a plan (no counterpart in the reference; the autotuner in
  * leopard_em_amd.backend.autotune sets them).  Keys: "colpass_columns" (1 or 2 columns
  * per column-pass block), "colpass_pair" (0/1: two correlations per column-pass
@@ -307,6 +317,27 @@ int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, i
                        void* image_dft, float* whitening_template, void* workspace, void* stream);
 
 /*
+ * tdtm_prepare_image with the bandpass and arbitrary-curve filters of the pre-processing
+ * (config/correlation_filters.py:164-334, combined at :374-410; phase randomisation is not
+ * part of the combined filter there, :395-396).  Filters on |k| in cycles / pixel:
+ *   bandpass (bandpass != 0): 1 on bp_low <= |k| <= bp_high, a raised-cosine edge
+ *   0.5 (1 + cos(pi d / bp_falloff)) at distance 0 < d <= bp_falloff outside the band, 0
+ *   beyond (hard edges for bp_falloff = 0);
+ *   curve (n_curve > 0 points, host arrays, increasing frequencies, n_curve <= 64): linear
+ *   interpolation of curve_amp at |k|, 1 below the first / above the last frequency.
+ * The image spectrum is multiplied by whitening x bandpass x curve on its own grid, the
+ * bandpass sets the dimensionality sum(bp) + sum(bp[:, 1:-1]) of the normalisation
+ * (utils/image_processing.py:60-67, match_template_manager.py:163-165), and
+ * template_filter = whitening x bandpass x curve on the (h, h/2+1) grid.  Workspace of
+ * tdtm_prepare_image_filtered_workspace_bytes(H, W, num_freq_bins) device bytes.
+ */
+int64_t tdtm_prepare_image_filtered_workspace_bytes(int H, int W, int num_freq_bins);
+int tdtm_prepare_image_filtered(const float* image, int H, int W, int h, int whitening, int num_freq_bins,
+                                double max_freq, int bandpass, double bp_low, double bp_high, double bp_falloff,
+                                int n_curve, const double* curve_freq, const double* curve_amp, void* image_dft,
+                                float* template_filter, void* workspace, void* stream);
+
+/*
  * tdtm_prepare_image without any device allocation.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  Same outputs as tdtm_prepare_image,
@@ -333,6 +364,17 @@ int tdtm_prepare_image_noalloc(const float* image, int H, int W, int h, int whit
 int tdtm_ctf_filter_stack(int h, int num_defocus, int num_pixel, double defocus_u, double defocus_v,
                           double defocus_angle, const float* defocus_offsets, const float* pixel_offsets,
                           const double* ctf_params, float* out, void* stream);
+
+/*
+ * tdtm_ctf_filter_stack with an anisotropic magnification matrix (host doubles
+ * {a00, a01, a10, a11}, NULL = none): the CTF at rfft frequency k = (kx, ky) is evaluated at
+ * A k, A acting on (x, y) column vectors; the B-envelope stays at k.  Replaces the
+ * transform_matrix argument of calculate_ctf_2d (src/leopard_em/utils/ctf_utils.py:66-100,
+ * OpticsGroup.mag_matrix_tensor).
+ */
+int tdtm_ctf_filter_stack_mag(int h, int num_defocus, int num_pixel, double defocus_u, double defocus_v,
+                              double defocus_angle, const float* defocus_offsets, const float* pixel_offsets,
+                              const double* ctf_params, const double* mag_matrix, float* out, void* stream);
 
 #ifdef __cplusplus
 }

@@ -81,36 +81,14 @@ __device__ __forceinline__ void rotmat_zyz(const float* e, float (&r)[9]) {
   r[8] = cth;
 }
 
-// R = Rz(phi) Ry(theta) Rz(psi) of orientation o0 + b, right-multiplied by the rotation
-// of `base` (refinement: offset composed with the particle's pose) when base != nullptr.
-// o0p != nullptr: the first orientation is read from device memory (graph replays of a
-// captured step advance it on the device).
-__global__ void k_slice(const float2* __restrict__ vol, int n, const float* __restrict__ euler, int o0, int nb,
-                        const float* __restrict__ base, float2* __restrict__ S, const int* __restrict__ o0p) {
-  if (o0p != nullptr) o0 = *o0p;
+// Value of the central slice of the fftshifted 3D rfft `vol` (n, n, n/2+1) at slice-plane
+// frequency (fx, fy) (cycles / sample) for the rotation with columns 0, 1 = (r00, r10, r20),
+// (r01, r11, r21): trilinear sample at R (fx, fy, 0), conjugated where the rotated
+// frequency has x < 0 (Hermitian fold), zero outside the volume.
+__device__ __forceinline__ float2 slice_value(const float2* __restrict__ vol, int n, const float (&ro)[9], float fx,
+                                              float fy) {
   const int wr = n / 2 + 1;
-  const long total = (long)nb * n * wr;
-  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= total) return;
-  const int kx = idx % wr;
-  const int r = (idx / wr) % n;
-  const int b = idx / ((long)wr * n);
-  float ro[9];
-  rotmat_zyz(euler + 3 * (o0 + b), ro);
-  if (base != nullptr) {
-    float rb[9], rc[9];
-    rotmat_zyz(base, rb);
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) rc[3 * i + j] = ro[3 * i] * rb[j] + ro[3 * i + 1] * rb[3 + j] + ro[3 * i + 2] * rb[6 + j];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) ro[i] = rc[i];
-  }
-  const float r00 = ro[0], r01 = ro[1], r10 = ro[3], r11 = ro[4], r20 = ro[6], r21 = ro[7];
-  const float fx = (float)kx / (float)n;
-  const float fy = (float)(r < n / 2 ? r : r - n) / (float)n;
-  float rx = r00 * fx + r01 * fy, ry = r10 * fx + r11 * fy, rz = r20 * fx + r21 * fy;
+  float rx = ro[0] * fx + ro[1] * fy, ry = ro[3] * fx + ro[4] * fy, rz = ro[6] * fx + ro[7] * fy;
   const bool flip = rx < 0.f;
   if (flip) {
     rx = -rx;
@@ -139,6 +117,77 @@ __global__ void k_slice(const float2* __restrict__ vol, int n, const float* __re
             w110 * v110.y + w111 * v111.y;
   }
   if (flip) val.y = -val.y;
+  return val;
+}
+
+// Anisotropic magnification of the slices (a 2 x 2 matrix A on (x, y) frequency column
+// vectors; `on` = 0: none).  Follows torch_fourier_slice.transform_slice_2d as called at
+// backend/cross_correlation.py:81-88: the slice value at k is the slice's bilinear sample at
+// A k on its own (fftshifted-in-y) grid, folded (negated and conjugated) where x' < 0, zero
+// outside the grid.
+struct SliceMag {
+  float a00, a01, a10, a11;
+  int on;
+};
+
+// R = Rz(phi) Ry(theta) Rz(psi) of orientation o0 + b, right-multiplied by the rotation
+// of `base` (refinement: offset composed with the particle's pose) when base != nullptr.
+// o0p != nullptr: the first orientation is read from device memory (graph replays of a
+// captured step advance it on the device).
+__global__ void k_slice(const float2* __restrict__ vol, int n, const float* __restrict__ euler, int o0, int nb,
+                        const float* __restrict__ base, float2* __restrict__ S, const int* __restrict__ o0p,
+                        SliceMag mag) {
+  if (o0p != nullptr) o0 = *o0p;
+  const int wr = n / 2 + 1;
+  const long total = (long)nb * n * wr;
+  const long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int kx = idx % wr;
+  const int r = (idx / wr) % n;
+  const int b = idx / ((long)wr * n);
+  float ro[9];
+  rotmat_zyz(euler + 3 * (o0 + b), ro);
+  if (base != nullptr) {
+    float rb[9], rc[9];
+    rotmat_zyz(base, rb);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) rc[3 * i + j] = ro[3 * i] * rb[j] + ro[3 * i + 1] * rb[3 + j] + ro[3 * i + 2] * rb[6 + j];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) ro[i] = rc[i];
+  }
+  const float fx = (float)kx / (float)n;
+  const float fy = (float)(r < n / 2 ? r : r - n) / (float)n;
+  float2 val;
+  if (!mag.on) {
+    val = slice_value(vol, n, ro, fx, fy);
+  } else {
+    // bilinear sample of the slice grid (rows iy <-> fy = (iy - n/2)/n, columns ix <->
+    // fx = ix/n) at A k
+    float tx = mag.a00 * fx + mag.a01 * fy, ty = mag.a10 * fx + mag.a11 * fy;
+    const bool fold = tx < 0.f;
+    if (fold) {
+      tx = -tx;
+      ty = -ty;
+    }
+    const float cy = ty * n + (float)(n / 2), cx = tx * n;
+    val = make_float2(0.f, 0.f);
+    if (cy >= 0.f && cy <= (float)(n - 1) && cx >= 0.f && cx <= (float)(wr - 1)) {
+      const int y0 = (int)floorf(cy), x0 = (int)floorf(cx);
+      const float wy = cy - y0, wx = cx - x0;
+      const int y1 = min(y0 + 1, n - 1), x1 = min(x0 + 1, wr - 1);
+      const float inv = 1.f / (float)n;
+      const float fy0 = (float)(y0 - n / 2) * inv, fy1 = (float)(y1 - n / 2) * inv;
+      const float fx0 = (float)x0 * inv, fx1 = (float)x1 * inv;
+      const float2 s00 = slice_value(vol, n, ro, fx0, fy0), s01 = slice_value(vol, n, ro, fx1, fy0);
+      const float2 s10 = slice_value(vol, n, ro, fx0, fy1), s11 = slice_value(vol, n, ro, fx1, fy1);
+      const float w00 = (1 - wy) * (1 - wx), w01 = (1 - wy) * wx, w10 = wy * (1 - wx), w11 = wy * wx;
+      val.x = w00 * s00.x + w01 * s01.x + w10 * s10.x + w11 * s11.x;
+      val.y = w00 * s00.y + w01 * s01.y + w10 * s10.y + w11 * s11.y;
+    }
+    if (fold) val.y = -val.y;
+  }
   if (r == 0 && kx == 0) val = make_float2(0.f, 0.f);
   // contrast flip (-1) times (-1)^(ky+kx): the spatial quadrant swap of the projection
   const float sgn = ((r + kx) & 1) ? 1.f : -1.f;
@@ -1119,10 +1168,18 @@ __global__ void k_finalize(long n, double total, const float* __restrict__ mip, 
 
 
 // ------------------------------------------------------------------ refinement
+// Magnification matrix of the CTF frequencies (on = 0: none).
+struct CtfMag {
+  double a00, a01, a10, a11;
+  int on;
+};
+
 // CTF x B-envelope stack for one particle: (C, D, h, h/2+1), torch_ctf sign convention.
+// `mag` (on): the CTF at frequency k is evaluated at A k (anisotropic magnification,
+// utils/ctf_utils.py:66-100), the B-envelope at k (:103-111).
 __global__ void k_ctf_stack(int h, int D, int C, double df_mean, double astig, double astig_ang,
                             const float* __restrict__ dfo, const float* __restrict__ pxo, double lam, double cs_a,
-                            double px, double wac, double phase, double bfac, float* __restrict__ out) {
+                            double px, double wac, double phase, double bfac, CtfMag mag, float* __restrict__ out) {
   const int wr = h / 2 + 1;
   const long plane = (long)h * wr;
   const long total = plane * D * C;
@@ -1132,15 +1189,18 @@ __global__ void k_ctf_stack(int h, int D, int C, double df_mean, double astig, d
   const int r = (e / wr) % h;
   const int d = (e / plane) % D;
   const int c = e / (plane * D);
-  const double ky = (double)(r < (h + 1) / 2 ? r : r - h) / (double)h / px;
-  const double kxx = (double)kx / (double)h / px;
+  const double ky0 = (double)(r < (h + 1) / 2 ? r : r - h) / (double)h / px;
+  const double kx0 = (double)kx / (double)h / px;
+  const double ky = mag.on ? mag.a10 * kx0 + mag.a11 * ky0 : ky0;
+  const double kxx = mag.on ? mag.a00 * kx0 + mag.a01 * ky0 : kx0;
   const double k2 = ky * ky + kxx * kxx;
   const double ang = atan2(ky, kxx);
   const double ratio = (px + (double)pxo[c]) / px;
   const double cs = cs_a / (ratio * ratio * ratio * ratio);
   const double df = (double)dfo[d] + df_mean + astig * cos(2.0 * (ang - astig_ang));
   const double chi = -M_PI * lam * k2 * df + 0.5 * M_PI * cs * lam * lam * lam * k2 * k2 - wac - phase;
-  out[e] = (float)(-sin(chi) * exp(-bfac * k2 / 4.0));
+  const double k2e = ky0 * ky0 + kx0 * kx0;
+  out[e] = (float)(-sin(chi) * exp(-bfac * k2e / 4.0));
 }
 
 struct RefineBest {
@@ -1360,7 +1420,7 @@ __global__ void k_refine_select(const RefineBest* __restrict__ slots, int nbatch
 namespace tdtm {
 static int launch_ctf_stack(int h, int D, int C, double defocus_u, double defocus_v, double defocus_angle,
                             const float* defocus_offsets, const float* pixel_offsets, const double* ctf_params,
-                            float* out, hipStream_t st) {
+                            const double* mag, float* out, hipStream_t st) {
   const double voltage = ctf_params[0], cs_mm = ctf_params[1], ac = ctf_params[2], phase = ctf_params[3];
   const double px = ctf_params[4], bfac = ctf_params[5];
   if (!(voltage > 0) || !(px > 0) || !(ac >= 0 && ac < 1)) return fail("invalid CTF parameters");
@@ -1370,11 +1430,13 @@ static int launch_ctf_stack(int h, int D, int C, double defocus_u, double defocu
                                (1 + 1.602176634e-19 * v / (2 * 9.1093837015e-31 * 299792458.0 * 299792458.0))) *
                      1e10;
   const long tot = (long)h * (h / 2 + 1) * D * C;
+  CtfMag cm{1.0, 0.0, 0.0, 1.0, 0};
+  if (mag) cm = CtfMag{mag[0], mag[1], mag[2], mag[3], 1};
   k_ctf_stack<<<(tot + 255) / 256, 256, 0, st>>>(h, D, C, 0.5 * (defocus_u + defocus_v),
                                                  0.5 * std::fabs(defocus_u - defocus_v), defocus_angle * M_PI / 180.0,
                                                  defocus_offsets, pixel_offsets, lam, cs_mm * 1e7, px,
                                                  std::atan(ac / std::sqrt(1 - ac * ac)), phase * M_PI / 180.0, bfac,
-                                                 out);
+                                                 cm, out);
   TDTM_LAUNCHED();
   return 0;
 }
@@ -1405,6 +1467,10 @@ struct tdtm_match_plan {
   int opt_phase_timing = 0;     // record HIP events between the phases of every step
   int opt_fft_engine = 0;       // column transforms: 0 FP32 VALU; MFMA (H = 4096) 1 fp16 split data,
                                 // 2 bf16 split data, 3 fp16 rounded once per stage, 4 bf16 rounded once
+  // anisotropic magnification of the slices and of the refinement CTF
+  // (tdtm_match_plan_set_magnification; off by default)
+  SliceMag mag{1.f, 0.f, 0.f, 1.f, 0};
+  double mag_d[4] = {1.0, 0.0, 0.0, 1.0};  // the same matrix in double (CTF)
   // per-step events at the phase boundaries (project | rowfwd | colpass | rowpass), read and
   // released by tdtm_match_plan_phase_times
   std::vector<std::array<hipEvent_t, TDTM_NUM_PHASES + 1>> phase_events;
@@ -1785,7 +1851,16 @@ int tdtm_ctf_filter_stack(int h, int num_defocus, int num_pixel, double defocus_
   if (!defocus_offsets || !pixel_offsets || !ctf_params || !out) return fail("null argument");
   if (h < 2 || (h & 1) || num_defocus < 1 || num_pixel < 1) return fail("bad sizes");
   return launch_ctf_stack(h, num_defocus, num_pixel, defocus_u, defocus_v, defocus_angle, defocus_offsets,
-                          pixel_offsets, ctf_params, out, static_cast<hipStream_t>(stream));
+                          pixel_offsets, ctf_params, nullptr, out, static_cast<hipStream_t>(stream));
+}
+
+int tdtm_ctf_filter_stack_mag(int h, int num_defocus, int num_pixel, double defocus_u, double defocus_v,
+                              double defocus_angle, const float* defocus_offsets, const float* pixel_offsets,
+                              const double* ctf_params, const double* mag_matrix, float* out, void* stream) {
+  if (!defocus_offsets || !pixel_offsets || !ctf_params || !out) return fail("null argument");
+  if (h < 2 || (h & 1) || num_defocus < 1 || num_pixel < 1) return fail("bad sizes");
+  return launch_ctf_stack(h, num_defocus, num_pixel, defocus_u, defocus_v, defocus_angle, defocus_offsets,
+                          pixel_offsets, ctf_params, mag_matrix, out, static_cast<hipStream_t>(stream));
 }
 
 const char* tdtm_last_error(void) { return last_error().c_str(); }
@@ -1871,6 +1946,25 @@ int tdtm_match_plan_set_image_size(tdtm_match_plan_t p, int image_h, int image_w
   return 0;
 }
 
+int tdtm_match_plan_set_magnification(tdtm_match_plan_t p, const double* mag_matrix) {
+  if (!p) return fail("null argument");
+  SliceMag m{1.f, 0.f, 0.f, 1.f, 0};
+  if (mag_matrix) {
+    for (int i = 0; i < 4; ++i)
+      if (!std::isfinite(mag_matrix[i])) return fail("magnification matrix entries must be finite");
+    m = SliceMag{(float)mag_matrix[0], (float)mag_matrix[1], (float)mag_matrix[2], (float)mag_matrix[3], 1};
+  }
+  // the captured steps carry the slice kernel's arguments
+  release_graph(p);
+  release_refine_graphs(p);
+  p->mag = m;
+  p->mag_d[0] = mag_matrix ? mag_matrix[0] : 1.0;
+  p->mag_d[1] = mag_matrix ? mag_matrix[1] : 0.0;
+  p->mag_d[2] = mag_matrix ? mag_matrix[2] : 0.0;
+  p->mag_d[3] = mag_matrix ? mag_matrix[3] : 1.0;
+  return 0;
+}
+
 int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) {
   if (!p || !key) return fail("null argument");
   const std::string k(key);
@@ -1929,6 +2023,11 @@ int tdtm_match_plan_set_option(tdtm_match_plan_t p, const char* key, int value) 
     if (!p->opt_phase_timing) release_phase_events(p);
   } else {
     return fail("unknown option '" + k + "'");
+  }
+  // captured steps keep the launch shapes they were recorded with: record again
+  if (k != "phase_timing") {
+    release_graph(p);
+    release_refine_graphs(p);
   }
   return 0;
 }
@@ -1993,7 +2092,7 @@ static int enqueue_step(tdtm_match_plan* p, const void* template_dft, const floa
   }
   const long ns = (long)count * p->h * p->wr;
   k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, euler, start, count, nullptr, p->S,
-                                             o0p);
+                                             o0p, p->mag);
   TDTM_LAUNCHED();
   // the projection kernels take their batch size from p->K; run them on exactly K
   const int Ksave = p->K;
@@ -2144,7 +2243,7 @@ static int refine_batch(tdtm_match_plan* p, const void* template_dft, const floa
   const int K = count * p->D * p->C;
   const long ns = (long)count * p->h * p->wr;
   k_slice<<<(ns + 255) / 256, 256, 0, st>>>((const float2*)template_dft, p->h, offsets, start, count, pose, p->S,
-                                             state);
+                                             state, p->mag);
   TDTM_LAUNCHED();
   const int Ksave = p->K;
   p->K = K;
@@ -2220,7 +2319,9 @@ static int refine_graph(tdtm_match_plan* p, const void* template_dft, const floa
 // CTF stack x projective filter of one particle into the plan's filters (D, C of the plan).
 static int refine_filters(tdtm_match_plan* p, double du, double dv, double da, const float* dfo, const float* pxo,
                           const double* ctf_params, const float* projective_filter, hipStream_t st) {
-  if (launch_ctf_stack(p->h, p->D, p->C, du, dv, da, dfo, pxo, ctf_params, p->ctfbuf, st)) return 1;
+  if (launch_ctf_stack(p->h, p->D, p->C, du, dv, da, dfo, pxo, ctf_params, p->mag.on ? p->mag_d : nullptr,
+                       p->ctfbuf, st))
+    return 1;
   const long plane = (long)p->h * p->wr;
   const long tot = plane * p->D * p->C;
   k_filters<<<(tot + 255) / 256, 256, 0, st>>>(p->ctfbuf, projective_filter, plane, tot, p->filt);

@@ -264,74 +264,149 @@ __global__ __launch_bounds__(1024) void k_whiten_curve(const double* __restrict_
 
 // whitening value at frequency (ky, kx): linear interpolation of the curve at
 // f / fm * (nb - 1), 1 beyond fm
-__device__ __forceinline__ float whiten_at(double ky, double kx, const double* __restrict__ curve, int nb,
-                                           double fm) {
+__device__ __forceinline__ double whiten_at_d(double ky, double kx, const double* __restrict__ curve, int nb,
+                                              double fm) {
   const double f = sqrt(ky * ky + kx * kx);
-  if (!(f <= fm)) return 1.f;
+  if (!(f <= fm)) return 1.0;
   const double pos = f / fm * (double)(nb - 1);
   const int i0 = min(max((int)floor(pos), 0), nb - 1);
   const int i1 = min(i0 + 1, nb - 1);
   const double t = pos - (double)i0;
-  return (float)(curve[i0] * (1.0 - t) + curve[i1] * t);
+  return curve[i0] * (1.0 - t) + curve[i1] * t;
+}
+
+// Bandpass and arbitrary-curve filters of the pre-processing (config/correlation_filters.py:
+// 164-334): bandpass (on) = 1 on lo <= |k| <= hi, a raised-cosine edge 0.5 (1 + cos(pi d / fo))
+// at distance 0 < d <= fo outside the band, 0 beyond (hard edges for fo = 0); curve (nc knots,
+// increasing frequencies) = linear interpolation at |k|, 1 below the first / above the last
+// knot.  |k| in cycles / pixel.
+constexpr int MAX_CURVE = 64;
+struct RadialFilters {
+  int bp_on;
+  double lo, hi, fo;
+  int nc;
+  double cf[MAX_CURVE], ca[MAX_CURVE];
+};
+
+__device__ __forceinline__ double bandpass_at(const RadialFilters& rf, double f) {
+  const double d = fmax(rf.lo - f, 0.0) + fmax(f - rf.hi, 0.0);
+  if (rf.fo > 0.0) return d <= rf.fo ? 0.5 * (1.0 + cos(M_PI * fmin(d / rf.fo, 1.0))) : 0.0;
+  return d == 0.0 ? 1.0 : 0.0;
+}
+
+__device__ __forceinline__ double curve_at(const RadialFilters& rf, double f) {
+  if (!(f >= rf.cf[0] && f <= rf.cf[rf.nc - 1])) return 1.0;
+  if (rf.nc == 1) return rf.ca[0];
+  int j = 1;  // first knot above f (the last segment for f at the top knot)
+  while (j < rf.nc - 1 && rf.cf[j] <= f) ++j;
+  const double t = (f - rf.cf[j - 1]) / (rf.cf[j] - rf.cf[j - 1]);
+  return rf.ca[j - 1] + t * (rf.ca[j] - rf.ca[j - 1]);
+}
+
+// product of the enabled radial filters at |k| = f
+__device__ __forceinline__ double radial_at(const RadialFilters& rf, double f) {
+  double v = 1.0;
+  if (rf.bp_on) v *= bandpass_at(rf, f);
+  if (rf.nc > 0) v *= curve_at(rf, f);
+  return v;
+}
+
+__device__ __forceinline__ float whiten_at(double ky, double kx, const double* __restrict__ curve, int nb,
+                                           double fm) {
+  return (float)whiten_at_d(ky, kx, curve, nb, fm);
+}
+
+// whitening x radial filters (rf.bp_on == 0 and rf.nc == 0: the whitening value alone)
+__device__ __forceinline__ float filter_at(double ky, double kx, const double* __restrict__ curve, int nb, double fm,
+                                           const RadialFilters& rf) {
+  double v = whiten_at_d(ky, kx, curve, nb, fm);
+  if (rf.bp_on || rf.nc > 0) v *= radial_at(rf, sqrt(ky * ky + kx * kx));
+  return (float)v;
 }
 
 __global__ void k_whiten_eval(const double* __restrict__ curve, int nb, double fm, int h, int w,
-                              float* __restrict__ out) {
+                              RadialFilters rf, float* __restrict__ out) {
   const int wr = w / 2 + 1;
   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)h * wr) return;
   const int iy = (int)(e / wr), jx = (int)(e % wr);
-  out[e] = whiten_at(fftfreq(iy, h), rfftfreq(jx, w), curve, nb, fm);
+  out[e] = filter_at(fftfreq(iy, h), rfftfreq(jx, w), curve, nb, fm, rf);
 }
 
-// X *= whitening(ky, kx) in place (complex64 x float32, as utils/image_processing.py:49)
-// and per-block partial sums of |X|^2 with the inner columns 1 .. W/2-1 counted twice
-// (the energy of the full spectrum, :54-58).
+// X *= filter(ky, kx) in place (complex64 x float32, as utils/image_processing.py:49; the
+// filter is whitening x bandpass x curve) and per-block partial sums of |X|^2 with the inner
+// columns 1 .. W/2-1 counted twice (the energy of the full spectrum, :54-58); with the bandpass
+// on (part_dim != nullptr) also partial sums of the bandpass alone, inner columns twice (its
+// dimensionality, :66).
 __global__ __launch_bounds__(FILTER_BLOCK) void k_prep_filter(float2* __restrict__ X, int H, int W,
                                                               const double* __restrict__ curve, int nb, double fm,
-                                                              double* __restrict__ part) {
-  __shared__ double ss[FILTER_BLOCK];
+                                                              RadialFilters rf, double* __restrict__ part,
+                                                              double* __restrict__ part_dim) {
+  __shared__ double ss[FILTER_BLOCK], sd[FILTER_BLOCK];
   const int wr = W / 2 + 1;
   const long n = (long)H * wr;
   const long base = (long)blockIdx.x * FILTER_BLOCK * FILTER_PER_THREAD;
-  double s = 0.0;
+  double s = 0.0, sdim = 0.0;
 #pragma unroll 4
   for (int i = 0; i < FILTER_PER_THREAD; ++i) {
     const long e = base + (long)i * FILTER_BLOCK + threadIdx.x;
     if (e < n) {
       const int iy = (int)(e / wr), jx = (int)(e % wr);
-      const float wv = whiten_at(fftfreq(iy, H), rfftfreq(jx, W), curve, nb, fm);
+      const double ky = fftfreq(iy, H), kx = rfftfreq(jx, W);
+      const float wv = filter_at(ky, kx, curve, nb, fm, rf);
       float2 v = X[e];
       v = make_float2(v.x * wv, v.y * wv);
       X[e] = v;
       const double a = (double)v.x * v.x + (double)v.y * v.y;
-      s += (jx >= 1 && jx <= W / 2 - 1) ? 2.0 * a : a;
+      const bool inner = jx >= 1 && jx <= W / 2 - 1;
+      s += inner ? 2.0 * a : a;
+      if (part_dim) {
+        const double b = (double)(float)bandpass_at(rf, sqrt(ky * ky + kx * kx));
+        sdim += inner ? 2.0 * b : b;
+      }
     }
   }
   ss[threadIdx.x] = s;
+  sd[threadIdx.x] = sdim;
   __syncthreads();
   for (int w = FILTER_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) ss[threadIdx.x] += ss[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = ss[0];
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = ss[0];
+    if (part_dim) part_dim[blockIdx.x] = sd[0];
+  }
 }
 
-// X *= sqrt(dimensionality / energy) with dimensionality H W (an all-pass bandpass:
-// sum(bp) + sum(bp[:, 1:-1]), :60-67) and the box factor 1 (full image, :69-73).
-// Every block reduces the partial sums itself, in the same order.
+// X *= sqrt(dimensionality / energy) with dimensionality sum(bp) + sum(bp[:, 1:-1])
+// (:60-67): H W for an all-pass bandpass, else the sum of the partial sums part_dim; box
+// factor 1 (full image, :69-73).  Every block reduces the partial sums itself, in the same
+// order.
 __global__ __launch_bounds__(FILTER_BLOCK) void k_prep_scale(float2* __restrict__ X, long n,
                                                              const double* __restrict__ part, int npart,
-                                                             double dimensionality) {
-  __shared__ double ss[FILTER_BLOCK];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < npart; i += FILTER_BLOCK) s += part[i];
+                                                             double dimensionality,
+                                                             const double* __restrict__ part_dim) {
+  __shared__ double ss[FILTER_BLOCK], sd[FILTER_BLOCK];
+  double s = 0.0, sdim = 0.0;
+  for (int i = threadIdx.x; i < npart; i += FILTER_BLOCK) {
+    s += part[i];
+    if (part_dim) sdim += part_dim[i];
+  }
   ss[threadIdx.x] = s;
+  sd[threadIdx.x] = sdim;
   __syncthreads();
   for (int w = FILTER_BLOCK / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) ss[threadIdx.x] += ss[threadIdx.x + w];
+    if ((int)threadIdx.x < w) {
+      ss[threadIdx.x] += ss[threadIdx.x + w];
+      sd[threadIdx.x] += sd[threadIdx.x + w];
+    }
     __syncthreads();
   }
+  if (part_dim) dimensionality = sd[0];
   const float scale = (float)(sqrt(dimensionality) / sqrt(ss[0]));
   const long base = (long)blockIdx.x * FILTER_BLOCK * FILTER_PER_THREAD;
   for (int i = 0; i < FILTER_PER_THREAD; ++i) {
@@ -521,6 +596,7 @@ int rfft2_impl(const float* image, int H, int W, float2* out, hipStream_t st) {
 
 struct PrepWorkspace {
   double *sums, *cnts, *curve, *part;
+  double* part_dim;  // bandpass partial sums (filtered preparation with the bandpass on), or null
   double2* dft;  // any-size images: the row transforms (H, W/2+1)
   int nb, npart;
 };
@@ -538,7 +614,9 @@ long filter_blocks(int H, int W) {
   return (n + FILTER_BLOCK * FILTER_PER_THREAD - 1) / (FILTER_BLOCK * FILTER_PER_THREAD);
 }
 
-PrepWorkspace carve(void* ws, int H, int W, int requested_bins) {
+// Workspace doubles: sums, counts, curve (nb each), energy partials (npart), with `dim`
+// the bandpass partials (npart), then (sizes that are not powers of two) the row transforms.
+PrepWorkspace carve(void* ws, int H, int W, int requested_bins, bool dim = false) {
   PrepWorkspace w;
   w.nb = num_bins(H, W, requested_bins);
   w.npart = (int)filter_blocks(H, W);
@@ -547,16 +625,19 @@ PrepWorkspace carve(void* ws, int H, int W, int requested_bins) {
   w.cnts = p + w.nb;
   w.curve = p + 2 * w.nb;
   w.part = p + 3 * w.nb;
-  const long used = 3L * w.nb + w.npart;
+  w.part_dim = dim ? w.part + w.npart : nullptr;
+  const long used = 3L * w.nb + (dim ? 2L : 1L) * w.npart;
   w.dft = reinterpret_cast<double2*>(p + ((used + 1) & ~1L));  // 16-byte aligned
   return w;
 }
 
 // The preparation chain on an existing twiddle table and workspace (shared by the
 // allocating entry point and the scratch-array one).
+// rf: bandpass / curve filters (both off in the unfiltered entry points).
 int prepare_image_impl(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
-                       float2* X, float* whitening_template, void* workspace, const float2* tw, hipStream_t st) {
-  const PrepWorkspace w = carve(workspace, H, W, num_freq_bins);
+                       float2* X, float* whitening_template, void* workspace, const float2* tw, hipStream_t st,
+                       const RadialFilters& rf) {
+  const PrepWorkspace w = carve(workspace, H, W, num_freq_bins, rf.bp_on != 0);
   const double fm = max_freq;
   if (prep_pow2(H, W)) {
     if (rfft2_tw(image, H, W, X, tw, st)) return 1;
@@ -578,11 +659,12 @@ int prepare_image_impl(const float* image, int H, int W, int h, int whitening, i
     TDTM_LAUNCHED();
   }
   const long nt = (long)h * (h / 2 + 1);
-  k_whiten_eval<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(w.curve, w.nb, fm, h, h, whitening_template);
+  k_whiten_eval<<<(unsigned)((nt + 255) / 256), 256, 0, st>>>(w.curve, w.nb, fm, h, h, rf, whitening_template);
   TDTM_LAUNCHED();
-  k_prep_filter<<<w.npart, FILTER_BLOCK, 0, st>>>(X, H, W, w.curve, w.nb, fm, w.part);
+  k_prep_filter<<<w.npart, FILTER_BLOCK, 0, st>>>(X, H, W, w.curve, w.nb, fm, rf, w.part, w.part_dim);
   TDTM_LAUNCHED();
-  k_prep_scale<<<w.npart, FILTER_BLOCK, 0, st>>>(X, (long)H * (W / 2 + 1), w.part, w.npart, (double)H * (double)W);
+  k_prep_scale<<<w.npart, FILTER_BLOCK, 0, st>>>(X, (long)H * (W / 2 + 1), w.part, w.npart, (double)H * (double)W,
+                                                  w.part_dim);
   TDTM_LAUNCHED();
   return 0;
 }
@@ -719,6 +801,14 @@ int64_t tdtm_prepare_image_workspace_bytes(int H, int W, int num_freq_bins) {
   return doubles * (int64_t)sizeof(double);
 }
 
+int64_t tdtm_prepare_image_filtered_workspace_bytes(int H, int W, int num_freq_bins) {
+  if (!prep_size_ok(H, W)) return -1;
+  if (num_freq_bins == 1 || num_freq_bins > 65536) return -1;
+  int64_t doubles = 3L * num_bins(H, W, num_freq_bins) + 2 * filter_blocks(H, W);
+  if (!prep_pow2(H, W)) doubles = ((doubles + 1) & ~1L) + 2L * H * (W / 2 + 1);
+  return doubles * (int64_t)sizeof(double);
+}
+
 int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, int num_freq_bins, double max_freq,
                        void* image_dft, float* whitening_template, void* workspace, void* stream) {
   if (prepare_image_checks(image, H, W, h, num_freq_bins, max_freq, image_dft, whitening_template)) return 1;
@@ -726,7 +816,40 @@ int tdtm_prepare_image(const float* image, int H, int W, int h, int whitening, i
   const float2* tw = nullptr;
   if (prep_pow2(H, W) && device_twiddles(&tw)) return 1;
   return prepare_image_impl(image, H, W, h, whitening, num_freq_bins, max_freq, static_cast<float2*>(image_dft),
-                            whitening_template, workspace, tw, static_cast<hipStream_t>(stream));
+                            whitening_template, workspace, tw, static_cast<hipStream_t>(stream), RadialFilters{});
+}
+
+int tdtm_prepare_image_filtered(const float* image, int H, int W, int h, int whitening, int num_freq_bins,
+                                double max_freq, int bandpass, double bp_low, double bp_high, double bp_falloff,
+                                int n_curve, const double* curve_freq, const double* curve_amp, void* image_dft,
+                                float* template_filter, void* workspace, void* stream) {
+  if (prepare_image_checks(image, H, W, h, num_freq_bins, max_freq, image_dft, template_filter)) return 1;
+  if (!workspace) return fail("null argument");
+  RadialFilters rf{};
+  if (bandpass) {
+    if (!(bp_low >= 0.0) || !(bp_high >= 0.0) || !(bp_falloff >= 0.0))
+      return fail("bandpass cut-offs and falloff must be >= 0");
+    rf.bp_on = 1;
+    rf.lo = bp_low;
+    rf.hi = bp_high;
+    rf.fo = bp_falloff;
+  }
+  if (n_curve < 0 || n_curve > MAX_CURVE)
+    return fail("the arbitrary curve takes at most " + std::to_string(MAX_CURVE) + " points");
+  if (n_curve > 0) {
+    if (!curve_freq || !curve_amp) return fail("null argument");
+    for (int i = 0; i < n_curve; ++i) {
+      if (!std::isfinite(curve_freq[i]) || !std::isfinite(curve_amp[i])) return fail("curve values must be finite");
+      if (i > 0 && !(curve_freq[i] > curve_freq[i - 1])) return fail("curve frequencies must increase");
+      rf.cf[i] = curve_freq[i];
+      rf.ca[i] = curve_amp[i];
+    }
+    rf.nc = n_curve;
+  }
+  const float2* tw = nullptr;
+  if (prep_pow2(H, W) && device_twiddles(&tw)) return 1;
+  return prepare_image_impl(image, H, W, h, whitening, num_freq_bins, max_freq, static_cast<float2*>(image_dft),
+                            template_filter, workspace, tw, static_cast<hipStream_t>(stream), rf);
 }
 
 int tdtm_prepare_image_noalloc(const float* image, int H, int W, int h, int whitening, int num_freq_bins,
@@ -747,7 +870,7 @@ int tdtm_prepare_image_noalloc(const float* image, int H, int W, int h, int whit
   TDTM_LAUNCHED();
   void* ws = static_cast<unsigned char*>(base) + NOALLOC_TW_BYTES;
   return prepare_image_impl(image, H, W, h, whitening, num_freq_bins, max_freq, static_cast<float2*>(image_dft),
-                            whitening_template, ws, tw, st);
+                            whitening_template, ws, tw, st, RadialFilters{});
 }
 
 }  // extern "C"

@@ -59,6 +59,13 @@ SIGNATURES = {
                                            c_void_p, c_void_p]),
     "tdtm_ctf_filter_stack": (c_int, [c_int, c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tdtm_ctf_filter_stack_mag": (c_int, [c_int, c_int, c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "tdtm_match_plan_set_magnification": (c_int, [c_void_p, c_void_p]),
+    "tdtm_prepare_image_filtered_workspace_bytes": (c_int64, [c_int, c_int, c_int]),
+    "tdtm_prepare_image_filtered": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, ctypes.c_double, c_int,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, c_int, c_void_p,
+                                            c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
 }
 
 

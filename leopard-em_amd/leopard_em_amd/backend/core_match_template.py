@@ -116,6 +116,12 @@ class MatchPlan:
         """Launch-shape option (see tdtm_match_plan_set_option); results are unaffected."""
         _lib.check(self.lib.tdtm_match_plan_set_option(self.handle, key.encode(), int(value)))
 
+    def set_magnification(self, mag_matrix) -> None:
+        """Anisotropic magnification matrix (2 x 2, or a flat list of 4 as in
+        ``OpticsGroup.mag_matrix``; None = off) of the slices and the refinement CTF
+        (``tdtm_match_plan_set_magnification``; ``cross_correlation.py:81-88``)."""
+        _lib.check(self.lib.tdtm_match_plan_set_magnification(self.handle, mag_array(mag_matrix)))
+
     PHASES = ("project", "rowfwd", "colpass", "rowpass")
 
     def phase_times(self) -> tuple[dict[str, float], int]:
@@ -240,6 +246,17 @@ class MatchPlan:
         return dict(zip(keys, outs), mip=maps[0])
 
 
+def mag_array(mag_matrix):
+    """Host doubles {a00, a01, a10, a11} of a 2 x 2 magnification matrix (tensor, nested or
+    flat list of 4, ``OpticsGroup.mag_matrix_tensor`` order) or None."""
+    if mag_matrix is None:
+        return None
+    m = torch.as_tensor(mag_matrix, dtype=torch.float64).detach().cpu().reshape(-1)
+    if m.numel() != 4:
+        raise ValueError(f"mag_matrix must have 4 entries (2 x 2), got {m.numel()}")
+    return (_lib.ctypes.c_double * 4)(*[float(v) for v in m])
+
+
 def _as_device_list(device) -> list[torch.device]:
     if isinstance(device, (list, tuple)):
         devs = [torch.device(d) for d in device]
@@ -274,8 +291,6 @@ def core_match_template(
     total_defocus.
     """
     _ = backend
-    if mag_matrix is not None:
-        raise NotImplementedError("anisotropic magnification (mag_matrix) is not supported by the tdtm backend")
     total_cc_per_batch = orientation_batch_size * defocus_values.shape[0] * pixel_values.shape[0]
     if num_cuda_streams > total_cc_per_batch:
         warnings.warn(
@@ -299,6 +314,8 @@ def core_match_template(
     for r, dev in enumerate(devs):
         lo, hi = r * share, min(O, (r + 1) * share)
         plan = MatchPlan(H, W, h, batch, D, C, dev)
+        if mag_matrix is not None:
+            plan.set_magnification(mag_matrix)
         engine = int(os.environ.get("TDTM_FFT_ENGINE", "0"))
         if engine and plan.Hg == 4096:
             plan.set_option("fft_engine", engine)
@@ -352,10 +369,10 @@ def _crc(t: torch.Tensor) -> int:
 
 
 def search_fingerprint(image_dft, template_dft, ctf_filters, whitening_filter_template, defocus_values,
-                       pixel_values, euler_angles, orientation_batch_size) -> dict:
-    """Identity of a search for checkpoint matching: shapes, batch size and CRC-32s of
-    every input."""
-    return {
+                       pixel_values, euler_angles, orientation_batch_size, mag_matrix=None) -> dict:
+    """Identity of a search for checkpoint matching: shapes, batch size, magnification and
+    CRC-32s of every input."""
+    fp = {
         "image_shape": list(image_dft.shape),
         "template_shape": list(template_dft.shape),
         "num_orientations": int(euler_angles.shape[0]),
@@ -365,6 +382,9 @@ def search_fingerprint(image_dft, template_dft, ctf_filters, whitening_filter_te
         "crc": [_crc(t) for t in (image_dft, template_dft, ctf_filters, whitening_filter_template,
                                   defocus_values, pixel_values, euler_angles)],
     }
+    if mag_matrix is not None:
+        fp["mag_matrix"] = [float(v) for v in torch.as_tensor(mag_matrix, dtype=torch.float64).reshape(-1)]
+    return fp
 
 
 # pylint: disable=too-many-arguments,too-many-locals
@@ -381,6 +401,7 @@ def core_match_template_resumable(
     orientation_batch_size: int = 1,
     checkpoint_interval_s: float = 600.0,
     max_batches: int | None = None,
+    mag_matrix: torch.Tensor | None = None,
 ) -> dict[str, torch.Tensor] | None:
     """``core_match_template`` on one device with checkpoint / resume of the statistics.
 
@@ -404,8 +425,10 @@ def core_match_template_resumable(
     O, D, C = euler_angles.shape[0], defocus_values.shape[0], pixel_values.shape[0]
     batch = max(1, min(orientation_batch_size, O))
     fp = search_fingerprint(image_dft, template_dft, ctf_filters, whitening_filter_template, defocus_values,
-                            pixel_values, euler_angles, batch)
+                            pixel_values, euler_angles, batch, mag_matrix)
     plan = MatchPlan(H, W, h, batch, D, C, dev)
+    if mag_matrix is not None:
+        plan.set_magnification(mag_matrix)
     with torch.cuda.device(dev):
         plan.set_inputs(image_dft, ctf_filters, whitening_filter_template)
         tdft = template_dft.to(dev, torch.complex64).contiguous()

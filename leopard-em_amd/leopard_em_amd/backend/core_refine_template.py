@@ -18,7 +18,7 @@ import ctypes
 import torch
 
 from .. import _lib
-from .core_match_template import MatchPlan, _as_device_list
+from .core_match_template import MatchPlan, _as_device_list, mag_array
 
 _CTF_KEYS = ("voltage", "spherical_aberration", "amplitude_contrast_ratio", "phase_shift", "pixel_size",
              "ctf_B_factor")
@@ -29,10 +29,18 @@ def _ctf_params(ctf_kwargs: dict) -> ctypes.Array:
         z = ctf_kwargs.get(k)
         if z:
             raise NotImplementedError(f"{k} are not supported by the tdtm backend")
-    if ctf_kwargs.get("mag_matrix") is not None:
-        raise NotImplementedError("anisotropic magnification is not supported by the tdtm backend")
     vals = [float(ctf_kwargs.get(k, 0.0)) for k in _CTF_KEYS]
     return (ctypes.c_double * 6)(*vals)
+
+
+def _magnification(ctf_kwargs: dict, mag_matrix):
+    """The one magnification matrix of slices and CTF: the ``mag_matrix`` argument
+    (``core_refine_template.py:856-868``) and ``ctf_kwargs["mag_matrix"]`` (``:650-660``),
+    which come from the same particle stack in the reference (``utils/backend_setup.py:618-641``)."""
+    a, b = mag_array(mag_matrix), mag_array(ctf_kwargs.get("mag_matrix"))
+    if a is not None and b is not None and list(a) != list(b):
+        raise ValueError("mag_matrix and ctf_kwargs['mag_matrix'] differ")
+    return a if a is not None else b
 
 
 def _int_array(vals) -> ctypes.Array:
@@ -45,11 +53,12 @@ def _double_array(vals) -> ctypes.Array:
 
 def _refine_stack(particle_stack_dft, template_dft, euler_angles, offsets, off_start, off_count, dfo, d_start,
                   d_count, pxo, c_start, c_count, defocus_u, defocus_v, defocus_angle, corr_mean, corr_std,
-                  ctf_kwargs, projective_filters, device, batch_size) -> dict[str, torch.Tensor]:
+                  ctf_kwargs, projective_filters, device, batch_size, mag_matrix=None) -> dict[str, torch.Tensor]:
     """One tdtm_refine_batch call over the stack; grids given as concatenated arrays with
     per-particle (start, count).  Returns the reference's output dictionary."""
     dev = _as_device_list(device)[0]
     params = _ctf_params(ctf_kwargs)
+    mag = _magnification(ctf_kwargs, mag_matrix)
     n, img_h, img_wr = particle_stack_dft.shape
     img_w = 2 * (img_wr - 1)
     th = template_dft.shape[1]
@@ -59,6 +68,8 @@ def _refine_stack(particle_stack_dft, template_dft, euler_angles, offsets, off_s
         raise ValueError(f"corr_mean / corr_std must have valid-crop shape {crop}")
     D, C = max(d_count), max(c_count)
     plan = MatchPlan(img_h, img_w, th, batch_size, D, C, dev)
+    if mag is not None:
+        plan.set_magnification(list(mag))
     lib = plan.lib
     with torch.cuda.device(dev):
         st = _lib.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
@@ -117,15 +128,13 @@ def core_refine_template(
     """Refine orientation, defocus, pixel size and position of every particle (one shared
     grid, as the reference; the whole stack in one tdtm_refine_batch call)."""
     _ = num_cuda_streams
-    if mag_matrix is not None:
-        raise NotImplementedError("anisotropic magnification (mag_matrix) is not supported by the tdtm backend")
     n = particle_stack_dft.shape[0]
     no, nd, nc = euler_angle_offsets.shape[0], defocus_offsets.shape[0], pixel_size_offsets.shape[0]
     return _refine_stack(particle_stack_dft, template_dft, euler_angles, euler_angle_offsets, [0] * n, [no] * n,
                          defocus_offsets, [0] * n, [nd] * n, pixel_size_offsets, [0] * n, [nc] * n,
                          [float(v) for v in defocus_u], [float(v) for v in defocus_v],
                          [float(v) for v in defocus_angle], corr_mean, corr_std, ctf_kwargs, projective_filters,
-                         device, batch_size)
+                         device, batch_size, mag_matrix)
 
 
 # pylint: disable=too-many-arguments,too-many-locals
@@ -145,6 +154,7 @@ def refine_template_ragged(
     projective_filters: torch.Tensor,
     device: torch.device | list[torch.device],
     batch_size: int = 32,
+    mag_matrix: torch.Tensor | None = None,
 ) -> dict[str, torch.Tensor]:
     """``core_refine_template`` with a grid of its own for every particle: particle i
     searches ``euler_angle_offsets[i]`` (n_i, 3), ``defocus_offsets[i]`` and
@@ -167,4 +177,4 @@ def refine_template_ragged(
     return _refine_stack(particle_stack_dft, template_dft, euler_angles, offs.reshape(-1, 3), o_s, o_c, dfo, d_s,
                          d_c, pxo, c_s, c_c, [float(v) for v in defocus_u], [float(v) for v in defocus_v],
                          [float(v) for v in defocus_angle], corr_mean, corr_std, ctf_kwargs, projective_filters,
-                         device, batch_size)
+                         device, batch_size, mag_matrix)

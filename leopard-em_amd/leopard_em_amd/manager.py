@@ -5,11 +5,13 @@ Counterpart of ``MatchTemplateManager`` and ``MatchTemplateResult``
 ``pydantic_models/results/match_template_result.py:40-337``) for the configuration
 subset the search engine runs:
 
-* optics group (CTF parameters), relative defocus search, uniform orientation grid
-  with C1 symmetry or explicit angle ranges, whitening filter on or off;
-* bandpass / arbitrary-curve / phase-randomisation filters, anisotropic
-  magnification, Zernike terms, other base grids and symmetries raise
-  NotImplementedError when enabled;
+* optics group (CTF parameters, anisotropic magnification matrix), relative defocus
+  search, uniform orientation grid with C1 symmetry or explicit angle ranges, whitening,
+  bandpass and arbitrary-curve filters (the phase-randomisation configuration is
+  accepted and, as in the reference's combined filter, not applied:
+  ``config/correlation_filters.py:395-396``);
+* Zernike terms, other base grids and symmetries raise NotImplementedError when
+  enabled;
 * YAML keys, defaults and output names follow the reference's example configuration
   (``programs/match_template/match_template_example_config.yaml``); plain classes
   stand in for the pydantic models.
@@ -37,8 +39,7 @@ from .analysis import (
 )
 from .backend.core_match_template import core_match_template, core_match_template_resumable
 from .io.mrc import load_mrc_image, load_mrc_volume, read_mrc_header, write_mrc_from_tensor
-from .utils.preprocessing import (ctf_filter_stack, prepare_image_device, preprocess_image,
-                                  volume_to_rfft_fourier_slice, whitening_filter)
+from .utils.preprocessing import ctf_filter_stack, prepare_image, volume_to_rfft_fourier_slice
 from .utils.search import get_search_tensors, get_uniform_euler_angles
 
 
@@ -62,8 +63,16 @@ class OpticsGroup:
             setattr(self, k, v)
         if self.odd_zernikes or self.even_zernikes:
             raise NotImplementedError("Zernike aberration terms are not supported by the tdtm backend")
-        if self.mag_matrix is not None:
-            raise NotImplementedError("anisotropic magnification (mag_matrix) is not supported by the tdtm backend")
+        if self.mag_matrix is not None and len(self.mag_matrix) != 4:
+            raise ValueError("mag_matrix must be a list of 4 floats")
+
+    @property
+    def mag_matrix_tensor(self) -> Optional[torch.Tensor]:
+        """2 x 2 float32 tensor [[m0, m1], [m2, m3]] of ``mag_matrix`` or None
+        (``optics_group.py:97-113``)."""
+        if self.mag_matrix is None:
+            return None
+        return torch.tensor(self.mag_matrix, dtype=torch.float32).reshape(2, 2)
 
 
 class DefocusSearchConfig:
@@ -115,7 +124,9 @@ class OrientationSearchConfig:
 
 
 class PreprocessingFilters:
-    """``config/correlation_filters.py``: whitening only (other filters must be off)."""
+    """``config/correlation_filters.py``: whitening, bandpass and arbitrary-curve filters
+    (phase randomisation is configured but left out of the combined filter, as in the
+    reference, ``:395-396``)."""
 
     def __init__(self, **cfg):
         known = {"whitening_filter", "bandpass_filter", "phase_randomization_filter", "arbitrary_curve_filter"}
@@ -129,17 +140,23 @@ class PreprocessingFilters:
         self.max_freq = 0.5 if wf["max_freq"] is None else float(wf["max_freq"])
         if not wf["do_power_spectrum"]:
             raise NotImplementedError("whitening from the amplitude spectrum (do_power_spectrum=False)")
-        for name in ("bandpass_filter", "phase_randomization_filter", "arbitrary_curve_filter"):
-            sub = cfg.get(name) or {}
-            if sub.get("enabled", False):
-                raise NotImplementedError(f"{name} is not supported by the tdtm harness")
-
-    def filter_for(self, image_dft: torch.Tensor, output_shape) -> torch.Tensor:
-        if not self.whitening_enabled:
-            return torch.ones(output_shape, dtype=torch.float32)
-        return whitening_filter(image_dft, tuple(output_shape), self.num_freq_bins, self.max_freq)
-
-
+        bp = _take(cfg.get("bandpass_filter") or {}, "BandpassFilterConfig",
+                   dict(enabled=False, low_freq_cutoff=None, high_freq_cutoff=None, falloff=None))
+        # None -> the reference's defaults (correlation_filters.py:256-259)
+        self.bandpass = None
+        if bp["enabled"]:
+            self.bandpass = (0.0 if bp["low_freq_cutoff"] is None else float(bp["low_freq_cutoff"]),
+                             1.0 if bp["high_freq_cutoff"] is None else float(bp["high_freq_cutoff"]),
+                             0.0 if bp["falloff"] is None else float(bp["falloff"]))
+        ac = _take(cfg.get("arbitrary_curve_filter") or {}, "ArbitraryCurveFilterConfig",
+                   dict(enabled=False, frequencies=None, amplitudes=None))
+        self.curve = None
+        if ac["enabled"]:
+            if ac["frequencies"] is None or ac["amplitudes"] is None:
+                raise ValueError("When enabled, both 'frequencies' and 'amplitudes' must be provided.")
+            self.curve = ([float(v) for v in ac["frequencies"]], [float(v) for v in ac["amplitudes"]])
+        self.phase_randomization = _take(cfg.get("phase_randomization_filter") or {},
+                                         "PhaseRandomizationFilterConfig", dict(enabled=False, cuton=None))
 class ComputationalConfig:
     """``config/computational_config.py``: devices and the stream-count hint."""
 
@@ -295,21 +312,14 @@ class MatchTemplateManager:
         # with a GPU the inputs are built by the library's kernels on the first search
         # device; otherwise on the host with torch, as the reference does
         dev = self.computational_config.gpu_devices[0] if torch.cuda.is_available() else torch.device("cpu")
-        if dev.type == "cuda":
-            image_pre, filt_tmp = prepare_image_device(image.to(dev), template.shape[-1], pf.whitening_enabled,
-                                                       pf.num_freq_bins, pf.max_freq)
-        else:
-            image_dft = torch.fft.rfftn(image)  # pylint: disable=not-callable
-            image_dft[0, 0] = 0
-            bandpass = torch.ones(image_dft.shape, dtype=torch.float32)
-            filt_img = pf.filter_for(image_dft, image_dft.shape)
-            filt_tmp = pf.filter_for(image_dft, (template.shape[-2], template.shape[-1] // 2 + 1))
-            image_pre = preprocess_image(image_dft, filt_img, bandpass, tuple(image.shape), tuple(image.shape))
+        image_pre, filt_tmp = prepare_image(image.to(dev), template.shape[-1], whitening=pf.whitening_enabled,
+                                            num_freq_bins=pf.num_freq_bins, max_freq=pf.max_freq,
+                                            bandpass=pf.bandpass, curve=pf.curve)
         ctf = ctf_filter_stack(
             (template.shape[0], template.shape[0]), og.defocus_u, og.defocus_v, og.astigmatism_angle,
             defocus_values, pixel_size_offsets, voltage=og.voltage, spherical_aberration=og.spherical_aberration,
             amplitude_contrast_ratio=og.amplitude_contrast_ratio, phase_shift=og.phase_shift,
-            pixel_size=og.pixel_size, ctf_B_factor=og.ctf_B_factor, device=dev)
+            pixel_size=og.pixel_size, ctf_B_factor=og.ctf_B_factor, mag_matrix=og.mag_matrix_tensor, device=dev)
         return {
             "image_dft": image_pre.to(torch.complex64),
             "template_dft": volume_to_rfft_fourier_slice(template.to(dev)).to(torch.complex64),
@@ -319,6 +329,7 @@ class MatchTemplateManager:
             "defocus_values": defocus_values,
             "pixel_values": pixel_size_offsets,
             "device": self.computational_config.gpu_devices,
+            "mag_matrix": og.mag_matrix_tensor,
         }
 
     def run_match_template(self, orientation_batch_size: int = 16, do_result_export: bool = True,
@@ -397,7 +408,7 @@ class MatchTemplateManager:
             df[k] = getattr(og, k)
         df["even_zernikes"] = [None] * len(df)
         df["odd_zernikes"] = [None] * len(df)
-        df["mag_matrix"] = [None] * len(df)
+        df["mag_matrix"] = [og.mag_matrix] * len(df)
         df["micrograph_path"] = self.micrograph_path
         df["template_path"] = self.template_volume_path
         df["mip_path"] = r.mip_path

@@ -8,6 +8,9 @@ Restates:
   ``config/correlation_filters.py:49-108``.
 * ``utils/image_processing.py:16-75`` (``preprocess_image``) and ``:179-202``
   (``volume_to_rfft_fourier_slice``).
+* ``torch_fourier_filter.bandpass.bandpass_filter`` and
+  ``torch_fourier_filter.utils.curve_1dim_to_ndim`` as wrapped by
+  ``config/correlation_filters.py:164-334`` (bandpass and arbitrary-curve filters).
 
 The CTF sign convention follows torch_ctf: CTF(k) = -sin(k1 n4 df + k2 n4^2 - w - phi)
 with k1 = -pi*lambda, k2 = pi/2*Cs*lambda^3, w = atan(A/sqrt(1-A^2)), so CTF(0) = +A;
@@ -57,11 +60,16 @@ def calculate_ctf_2d(
     pixel_size: float,
     image_shape: tuple[int, int],
     dtype=torch.float64,
+    transform_matrix: torch.Tensor | None = None,
 ) -> torch.Tensor:
     """CTF on the unshifted rfft grid, one plane per defocus value -> (D, h, w//2+1).
 
     Astigmatic defocus df(alpha) = df + astig * cos(2 (alpha - alpha_ast)) with
     alpha = atan2(ky, kx) (torch_ctf's convention, defocus_u = df + astig).
+    ``transform_matrix`` (anisotropic magnification, ``utils/ctf_utils.py:66-80``,
+    ``:97-100``): the CTF at frequency k = (kx, ky) is evaluated at A k, A acting on
+    (x, y) column vectors as in :func:`oracle.geometry.transform_slice_2d` (the upstream
+    order is parity unpinned).
     """
     defocus = torch.as_tensor(defocus_um, dtype=dtype).reshape(-1) * 1e4
     astig = float(astigmatism_um) * 1e4
@@ -73,6 +81,9 @@ def calculate_ctf_2d(
     k3 = math.atan(amplitude_contrast / math.sqrt(1 - amplitude_contrast**2))
     k5 = math.radians(phase_shift_deg)
     ky, kx = rfft_freq_grid(image_shape, dtype)
+    if transform_matrix is not None:
+        a = torch.as_tensor(transform_matrix, dtype=dtype).reshape(2, 2)
+        kx, ky = a[0, 0] * kx + a[0, 1] * ky, a[1, 0] * kx + a[1, 1] * ky
     ky = ky / pixel_size
     kx = kx / pixel_size
     n4 = ky**2 + kx**2
@@ -102,12 +113,14 @@ def calculate_ctf_filter_stack_full_args(
     phase_shift: float,
     pixel_size: float,
     ctf_B_factor: float = 0.0,
+    mag_matrix: torch.Tensor | None = None,
     dtype=torch.float64,
 ) -> torch.Tensor:
     """``utils/ctf_utils.py:19-116`` -> (num_pixel_sizes, num_defocus, h, w//2+1).
 
     defocus = offsets + (u+v)/2, astigmatism = |u-v|/2 (``:51-52``), one Cs per pixel
-    size offset via ``get_cs_range`` (``:55-59``), B-envelope applied (``:103-111``).
+    size offset via ``get_cs_range`` (``:55-59``), the magnification matrix applied to
+    the CTF's frequencies (``:66-100``) but not to the B-envelope (``:103-111``).
     """
     defocus = torch.as_tensor(defocus_offsets, dtype=dtype) + (defocus_u + defocus_v) / 2
     astig = abs(defocus_u - defocus_v) / 2
@@ -117,7 +130,7 @@ def calculate_ctf_filter_stack_full_args(
     for cs in cs_values.tolist():
         ctf = calculate_ctf_2d(
             defocus * 1e-4, astig * 1e-4, astigmatism_angle, voltage, cs,
-            amplitude_contrast_ratio, phase_shift, pixel_size, template_shape, dtype,
+            amplitude_contrast_ratio, phase_shift, pixel_size, template_shape, dtype, mag_matrix,
         )
         planes.append(ctf * env[None])
     return torch.stack(planes, 0)
@@ -165,6 +178,56 @@ def whitening_filter(
     return torch.where(of <= max_freq, out, torch.ones_like(out))
 
 
+def radial_freq(shape_real: tuple[int, int], dtype=torch.float64) -> torch.Tensor:
+    """|k| in cycles / pixel on the unshifted rfft grid of an (h, w) real image
+    (torch_grid_utils ``fftfreq_grid(rfft=True, fftshift=False, norm=True)``)."""
+    ky, kx = rfft_freq_grid(shape_real, dtype)
+    return torch.sqrt(ky**2 + kx**2)
+
+
+def bandpass_filter(low: float, high: float, falloff: float, shape_real: tuple[int, int],
+                    dtype=torch.float64) -> torch.Tensor:
+    """Band-pass on the unshifted rfft grid of an (h, w) real image -> (h, w//2+1).
+
+    ``config/correlation_filters.py:264-271`` calls torch_fourier_filter's
+    ``bandpass_filter(low, high, falloff, image_shape, rfft=True, fftshift=False)`` with
+    cut-offs in cycles / pixel (``from_spatial_resolution``: pixel size / resolution,
+    ``:206-212``) and its defaults low 0, high 1, falloff 0 (``:256-259``).  Restated as
+    1 on low <= |k| <= high, a raised-cosine edge 0.5 (1 + cos(pi d / falloff)) at a
+    distance 0 < d <= falloff outside the band, 0 beyond (hard edges when falloff = 0).
+    The upstream edge shape is parity unpinned (the package is absent).
+    """
+    f = radial_freq(shape_real, dtype)
+    d = torch.clamp(low - f, min=0) + torch.clamp(f - high, min=0)  # distance outside the band
+    if falloff > 0:
+        edge = 0.5 * (1 + torch.cos(math.pi * torch.clamp(d / falloff, max=1.0)))
+        return torch.where(d <= falloff, edge, torch.zeros_like(f))
+    return (d == 0).to(dtype)
+
+
+def arbitrary_curve_filter(frequencies, amplitudes, shape_real: tuple[int, int], dtype=torch.float64) -> torch.Tensor:
+    """Radial curve on the unshifted rfft grid -> (h, w//2+1),
+    ``config/correlation_filters.py:281-334``: ``curve_1dim_to_ndim`` interpolates the
+    (frequency, amplitude) pairs linearly at every |k| (cycles / pixel, the same grid as
+    :func:`bandpass_filter`), with 1 below the first and above the last frequency
+    (``fill_lower`` = ``fill_upper`` = 1).  Frequencies must increase.
+    """
+    fq = torch.as_tensor(frequencies, dtype=dtype).reshape(-1)
+    am = torch.as_tensor(amplitudes, dtype=dtype).reshape(-1)
+    if fq.numel() != am.numel() or fq.numel() < 1:
+        raise ValueError("frequencies and amplitudes must be non-empty and of one length")
+    f = radial_freq(shape_real, dtype)
+    j = torch.clamp(torch.searchsorted(fq, f, right=True), 1, max(fq.numel() - 1, 1))
+    if fq.numel() == 1:
+        val = torch.full_like(f, float(am[0]))
+    else:
+        f0, f1 = fq[j - 1], fq[j]
+        t = (f - f0) / (f1 - f0)
+        val = am[j - 1] + t * (am[j] - am[j - 1])
+    inside = (f >= fq[0]) & (f <= fq[-1])
+    return torch.where(inside, val, torch.ones_like(f))
+
+
 def preprocess_image(
     image_rfft: torch.Tensor,
     cumulative_fourier_filters: torch.Tensor,
@@ -192,17 +255,41 @@ def volume_to_rfft_fourier_slice(volume: torch.Tensor) -> torch.Tensor:
     return torch.fft.fftshift(v, dim=(0, 1))
 
 
-def prepare_image_dft(image: torch.Tensor, template_shape_rfft: tuple[int, int]) -> tuple[torch.Tensor, torch.Tensor]:
+def prepare_image_dft(image: torch.Tensor, template_shape_rfft: tuple[int, int], *, whitening: bool = True,
+                      num_freq_bins: int | None = None, max_freq: float = 0.5,
+                      bandpass: tuple[float, float, float] | None = None,
+                      curve: tuple[list, list] | None = None) -> tuple[torch.Tensor, torch.Tensor]:
     """Manager-side image preparation, ``match_template_manager.py:160-186``.
 
-    Returns (pre-processed image rfft, template-grid whitening filter) with the
-    default filter configuration (whitening on, bandpass and curve filters off).
+    Returns (pre-processed image rfft, template-grid combined filter).  The combined
+    filter is whitening x bandpass x arbitrary curve (``correlation_filters.py:374-410``;
+    phase randomisation is left out there, ``:395-396``), evaluated on the image grid
+    for the image and on the template's rfft grid for the template; the bandpass alone
+    sets the dimensionality of ``preprocess_image`` (``:163-165``, ``image_processing.py:60-67``).
+    ``bandpass`` = (low, high, falloff) or None (off); ``curve`` = (frequencies,
+    amplitudes) or None (off).
     """
     dtype = torch.float64 if image.dtype == torch.float64 else torch.float32
     image_dft = torch.fft.rfftn(image)
     image_dft[0, 0] = 0
-    bandpass = torch.ones(image_dft.shape, dtype=dtype)
-    wf_image = whitening_filter(image_dft, image_dft.shape).to(dtype)
-    wf_template = whitening_filter(image_dft, template_shape_rfft).to(dtype)
-    pre = preprocess_image(image_dft, wf_image, bandpass, tuple(image.shape), tuple(image.shape))
+    shape_img = tuple(image.shape)
+    h, w2 = template_shape_rfft
+    shape_tpl = (h, 2 * (w2 - 1))
+
+    def combined(shape_real, rfft_shape):
+        if whitening:
+            wf = whitening_filter(image_dft, rfft_shape, num_freq_bins, max_freq)
+        else:
+            wf = torch.ones(rfft_shape, dtype=torch.float64)
+        if bandpass is not None:
+            wf = wf * bandpass_filter(*bandpass, shape_real)
+        if curve is not None:
+            wf = wf * arbitrary_curve_filter(curve[0], curve[1], shape_real)
+        return wf.to(dtype)
+
+    bp = (bandpass_filter(*bandpass, shape_img) if bandpass is not None
+          else torch.ones(image_dft.shape, dtype=torch.float64)).to(dtype)
+    wf_image = combined(shape_img, image_dft.shape)
+    wf_template = combined(shape_tpl, template_shape_rfft)
+    pre = preprocess_image(image_dft, wf_image, bp, shape_img, shape_img)
     return pre, wf_template

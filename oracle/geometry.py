@@ -5,6 +5,8 @@ Restates the third-party geometry the reference calls:
 * ``roma.rotmat_to_euler`` / composition -- ``backend/utils.py:25-54``
 * ``torch_fourier_slice.extract_central_slices_rfft_3d`` --
   ``backend/cross_correlation.py:76-79, 225-228, 330-333``
+* ``torch_fourier_slice.transform_slice_2d`` (anisotropic magnification) --
+  ``backend/cross_correlation.py:81-88, 231-238, 336-343``
 * ``torch_so3.get_uniform_euler_angles`` -- ``config/orientation_search.py:160-170``
 * ``get_search_tensors`` / ``get_cs_range`` -- ``utils/search_utils.py:6-93``
 """
@@ -138,6 +140,58 @@ def extract_central_slices_rfft_3d(volume_rfft: torch.Tensor, rotation_matrices:
     )
     samples = trilinear_sample_zero_outside(volume_rfft, zyx)
     return torch.where(flip, samples.conj(), samples)
+
+
+
+def bilinear_sample_zero_outside(image: torch.Tensor, coords_yx: torch.Tensor) -> torch.Tensor:
+    """Bilinear samples of an (h, w) array at array coordinates (..., 2) in yx order.
+
+    Same edge rule as :func:`trilinear_sample_zero_outside`: samples with a coordinate
+    outside [0, n-1] are zero, neighbours beyond the last index are clamped.
+    """
+    shape = torch.tensor(image.shape, dtype=coords_yx.dtype)
+    inside = ((coords_yx >= 0) & (coords_yx <= shape - 1)).all(-1)
+    c = torch.minimum(torch.maximum(coords_yx, torch.zeros_like(coords_yx)), shape - 1)
+    i0 = torch.floor(c).long()
+    frac = c - i0.to(c.dtype)
+    i1 = torch.minimum(i0 + 1, (shape - 1).long())
+    out = torch.zeros(coords_yx.shape[:-1], dtype=image.dtype)
+    for dy in (0, 1):
+        iy = i1[..., 0] if dy else i0[..., 0]
+        wy = frac[..., 0] if dy else 1 - frac[..., 0]
+        for dx in (0, 1):
+            ix = i1[..., 1] if dx else i0[..., 1]
+            wx = frac[..., 1] if dx else 1 - frac[..., 1]
+            out = out + (wy * wx).to(image.dtype) * image[iy, ix]
+    return torch.where(inside, out, torch.zeros_like(out))
+
+
+def transform_slice_2d(slices: torch.Tensor, transform_matrix: torch.Tensor) -> torch.Tensor:
+    """Anisotropic magnification of central slices (fftshifted-in-y rfft layout).
+
+    Follows torch_fourier_slice ``transform_slice_2d`` as called at
+    ``backend/cross_correlation.py:81-88`` (before the ifftshift / DC / contrast flip of
+    ``:90-92``): every output frequency k = (x, y) of the (n, n//2+1) slice grid takes
+    the input slice's value at k' = A k (A = ``transform_matrix``, 2 x 2, acting on
+    (x, y) column vectors, the row / column order of ``OpticsGroup.mag_matrix_tensor``);
+    points with x' < 0 are negated and their samples conjugated (Hermitian fold), and
+    the slice is sampled bilinearly at array coordinates (n//2 + y' n, x' n) with zero
+    outside.  Which of A or its inverse the upstream package applies, and its order of
+    (x, y), are not pinned by any file of the reference (the package is absent): this
+    restatement fixes k' = A k.
+    """
+    n = slices.shape[-2]
+    real_dtype = torch.float64 if slices.dtype == torch.complex128 else torch.float32
+    a = torch.as_tensor(transform_matrix, dtype=real_dtype).reshape(2, 2)
+    g = central_slice_fftfreq_grid(n, real_dtype)[..., :2]  # (n, n/2+1, 2) xy
+    t = torch.einsum("ij,hwj->hwi", a, g)
+    flip = t[..., 0] < 0
+    t = torch.where(flip[..., None], -t, t)
+    yx = torch.stack([t[..., 1] * n + n // 2, t[..., 0] * n], -1)
+    flat = slices.reshape(-1, n, n // 2 + 1)
+    out = torch.stack([bilinear_sample_zero_outside(flat[i], yx) for i in range(flat.shape[0])], 0)
+    out = out.reshape(slices.shape)
+    return torch.where(flip, out.conj(), out)
 
 
 # --------------------------------------------------------------------------- grids

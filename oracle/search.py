@@ -19,7 +19,7 @@ import numpy as np
 import torch
 
 from .filters import calculate_ctf_filter_stack_full_args
-from .geometry import combine_euler_angles, euler_to_rotmat_zyz, extract_central_slices_rfft_3d
+from .geometry import combine_euler_angles, euler_to_rotmat_zyz, extract_central_slices_rfft_3d, transform_slice_2d
 
 
 def normalize_template_projection(projections: torch.Tensor, small_shape, large_shape) -> torch.Tensor:
@@ -40,14 +40,17 @@ def normalize_template_projection(projections: torch.Tensor, small_shape, large_
 
 
 def make_projections(template_dft: torch.Tensor, rotation_matrices: torch.Tensor,
-                     projective_filters: torch.Tensor, image_shape) -> torch.Tensor:
-    """Slice -> ifftshift(y) -> DC=0 -> flip -> filter -> irfft2 -> ifftshift -> normalise.
+                     projective_filters: torch.Tensor, image_shape, mag_matrix=None) -> torch.Tensor:
+    """Slice -> [magnification] -> ifftshift(y) -> DC=0 -> flip -> filter -> irfft2 ->
+    ifftshift -> normalise.
 
     ``backend/cross_correlation.py:330-359``; returns (C, D, B, h, w).
     """
     h = template_dft.shape[1]
     w = 2 * (template_dft.shape[2] - 1)
     s = extract_central_slices_rfft_3d(template_dft, rotation_matrices)
+    if mag_matrix is not None:
+        s = transform_slice_2d(s, mag_matrix)
     s = torch.fft.ifftshift(s, dim=(-2,))
     s[..., 0, 0] = 0
     s = s * -1
@@ -58,10 +61,11 @@ def make_projections(template_dft: torch.Tensor, rotation_matrices: torch.Tensor
 
 
 def batched_cross_correlate(image_dft: torch.Tensor, template_dft: torch.Tensor,
-                            rotation_matrices: torch.Tensor, projective_filters: torch.Tensor) -> torch.Tensor:
+                            rotation_matrices: torch.Tensor, projective_filters: torch.Tensor,
+                            mag_matrix=None) -> torch.Tensor:
     """``backend/cross_correlation.py:287-369`` -> (C, D, B, H, W) correlation maps."""
     image_shape = (image_dft.shape[0], 2 * (image_dft.shape[1] - 1))
-    p = make_projections(template_dft, rotation_matrices, projective_filters, image_shape)
+    p = make_projections(template_dft, rotation_matrices, projective_filters, image_shape, mag_matrix)
     pd = torch.fft.rfftn(p, dim=(-2, -1), s=image_shape)
     pd[..., 0, 0] = 0
     pd = image_dft[None, None, None, ...] * pd.conj()
@@ -117,7 +121,7 @@ def scale_mip(mip, corr_sum, corr_sq_sum, total):
 
 def core_match_template(image_dft, template_dft, ctf_filters, whitening_filter_template,
                         defocus_values, pixel_values, euler_angles, orientation_batch_size=1,
-                        dtype=torch.float32, return_raw=False):
+                        dtype=torch.float32, return_raw=False, mag_matrix=None):
     """Single-device CPU restatement of ``backend/core_match_template.py:146-363``.
 
     The orientation loop follows ``:492-554`` with one device, so the work queue is
@@ -145,7 +149,7 @@ def core_match_template(image_dft, template_dft, ctf_filters, whitening_filter_t
         rot = euler_to_rotmat_zyz(eb, torch.float64 if dtype == torch.float64 else torch.float32)
         idx = torch.arange(i, i + eb.shape[0], dtype=torch.int32)
         bidx = (idx[None, :] + incr[:, None]).flatten()
-        cc = batched_cross_correlate(image_dft, template_dft, rot, filters)
+        cc = batched_cross_correlate(image_dft, template_dft, rot, filters, mag_matrix)
         do_iteration_statistics_updates(cc, bidx, mip, best, csum, csq)
     if return_raw:
         return {"mip": mip, "best_global_index": best, "correlation_sum": csum, "correlation_squared_sum": csq}
@@ -163,7 +167,7 @@ def core_match_template(image_dft, template_dft, ctf_filters, whitening_filter_t
 def refine_particle(particle_image_dft, template_dft, euler_angle, euler_angle_offsets,
                     defocus_u, defocus_v, defocus_angle, defocus_offsets, pixel_size_offsets,
                     corr_mean, corr_std, ctf_kwargs, projective_filter, batch_size=32,
-                    dtype=torch.float32):
+                    dtype=torch.float32, mag_matrix=None):
     """``backend/core_refine_template.py:555-767`` for one particle (CPU branch).
 
     Returns the dict of per-particle statistics the reference builds at ``:754-767``.
@@ -189,7 +193,7 @@ def refine_particle(particle_image_dft, template_dft, euler_angle, euler_angle_o
                    "ctf_B_factor")}
     ctf = calculate_ctf_filter_stack_full_args(
         (th, tw), float(defocus_u), float(defocus_v), defocus_offsets, pixel_size_offsets,
-        astigmatism_angle=float(defocus_angle), dtype=torch.float64, **kw).to(dtype)
+        astigmatism_angle=float(defocus_angle), mag_matrix=mag_matrix, dtype=torch.float64, **kw).to(dtype)
     filt = projective_filter.to(dtype)[None, None] * ctf
     nb = math.ceil(euler_angle_offsets.shape[0] / batch_size)
     for i in range(nb):
@@ -197,7 +201,7 @@ def refine_particle(particle_image_dft, template_dft, euler_angle, euler_angle_o
         e0 = min((i + 1) * batch_size, euler_angle_offsets.shape[0])
         ob = euler_angle_offsets[s0:e0]
         rot = euler_to_rotmat_zyz(ob, rdt) @ default_rot
-        cc = batched_cross_correlate(X, tdft, rot, filt)[..., :crop_h, :crop_w]
+        cc = batched_cross_correlate(X, tdft, rot, filt, mag_matrix)[..., :crop_h, :crop_w]
         z = (cc - corr_mean.to(dtype)) / corr_std.to(dtype)
         if z.max() > best["max_z_score"]:
             best["max_cc"] = float(cc.max())
@@ -221,8 +225,9 @@ def refine_particle(particle_image_dft, template_dft, euler_angle, euler_angle_o
 def core_refine_template(particle_stack_dft, template_dft, euler_angles, euler_angle_offsets,
                          defocus_offsets, defocus_u, defocus_v, defocus_angle, pixel_size_offsets,
                          corr_mean, corr_std, ctf_kwargs, projective_filters, batch_size=32,
-                         dtype=torch.float32):
-    """``backend/core_refine_template.py:33-202`` on one device, CPU branch."""
+                         dtype=torch.float32, mag_matrix=None):
+    """``backend/core_refine_template.py:33-202`` on one device, CPU branch; ``mag_matrix``
+    enters the slices and the per-particle CTF (``:856-868``, ``:650-660``)."""
     n = particle_stack_dft.shape[0]
     img_h = particle_stack_dft.shape[1]
     img_w = 2 * (particle_stack_dft.shape[2] - 1)
@@ -233,7 +238,7 @@ def core_refine_template(particle_stack_dft, template_dft, euler_angles, euler_a
         stats.append(refine_particle(
             particle_stack_dft[i], template_dft, euler_angles[i], euler_angle_offsets,
             defocus_u[i], defocus_v[i], defocus_angle[i], defocus_offsets, pixel_size_offsets,
-            corr_mean[i], corr_std[i], ctf_kwargs, projective_filters[i], batch_size, dtype))
+            corr_mean[i], corr_std[i], ctf_kwargs, projective_filters[i], batch_size, dtype, mag_matrix))
     ang = torch.empty((n, 3), dtype=torch.float32)
     for i, st in enumerate(stats):
         off = torch.tensor([st["refined_phi_offset"], st["refined_theta_offset"], st["refined_psi_offset"]],

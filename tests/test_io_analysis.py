@@ -306,3 +306,46 @@ def test_product_preprocessing_matches_oracle():
     assert float((ctf.double() - octf).abs().max()) < 1e-6
     vol = torch.from_numpy(np.random.default_rng(1).standard_normal((h, h, h)).astype(np.float32))
     assert torch.allclose(pp.volume_to_rfft_fourier_slice(vol), of.volume_to_rfft_fourier_slice(vol))
+
+
+def test_manager_filters_and_magnification_on_the_host():
+    """Bandpass / curve filters and the magnification matrix from the configuration reach
+    the search inputs (host path: CPU tensors), matching the oracle's manager chain
+    (match_template_manager.py:140-218); phase randomisation is accepted and not applied,
+    as in the reference's combined filter (correlation_filters.py:395-396)."""
+    from leopard_em_amd import synthetic
+    from oracle import filters as of
+
+    t = synthetic.make_blob_template(2, 32, 40, 10.0)
+    mg = synthetic.make_micrograph(t, 3, (128, 128), 2, 5200.0, 4950.0, 25.0, snr=0.5)
+    cfg = dict(REF_YAML)
+    cfg["optics_group"] = dict(REF_YAML["optics_group"], mag_matrix=[1.01, 0.02, -0.01, 0.99])
+    cfg["defocus_search_config"] = {"defocus_max": 200.0, "defocus_min": -200.0, "defocus_step": 200.0}
+    cfg["preprocessing_filters"] = {
+        "whitening_filter": {"enabled": True},
+        "bandpass_filter": {"enabled": True, "low_freq_cutoff": 0.02, "high_freq_cutoff": 0.4, "falloff": 0.05},
+        "arbitrary_curve_filter": {"enabled": True, "frequencies": [0.1, 0.3], "amplitudes": [1.0, 0.5]},
+        "phase_randomization_filter": {"enabled": True, "cuton": 0.4},
+    }
+    m = MatchTemplateManager.from_dict(dict(cfg, micrograph=mg.image, template_volume=t.volume()))
+    m.computational_config.gpu_ids = []  # host path even where a GPU exists
+    import torch as _t
+
+    if _t.cuda.is_available():
+        pytest.skip("host-path check (the manager builds inputs on the GPU when one is present)")
+    kw = m.make_backend_core_function_kwargs()
+    want, wf = of.prepare_image_dft(mg.image.double(), (32, 17), bandpass=(0.02, 0.4, 0.05),
+                                    curve=([0.1, 0.3], [1.0, 0.5]))
+    rel = float((kw["image_dft"].to(_t.complex128) - want).abs().max() / want.abs().max())
+    assert rel <= 1e-5
+    assert float((kw["whitening_filter_template"].double() - wf).abs().max()) <= 1e-6
+    mag = _t.tensor([[1.01, 0.02], [-0.01, 0.99]])
+    assert _t.equal(kw["mag_matrix"], mag)
+    ctf = of.calculate_ctf_filter_stack_full_args(
+        (32, 32), 5200.0, 4950.0, _t.tensor([-200.0, 0.0, 200.0]), _t.tensor([0.0]), astigmatism_angle=25.0,
+        voltage=300.0, spherical_aberration=2.7, amplitude_contrast_ratio=0.07, phase_shift=0.0, pixel_size=1.06,
+        ctf_B_factor=60.0, mag_matrix=mag.double())
+    assert float((kw["ctf_filters"].double() - ctf).abs().max()) <= 1e-6
+    bad = dict(cfg, preprocessing_filters={"arbitrary_curve_filter": {"enabled": True, "frequencies": [0.1]}})
+    with pytest.raises(ValueError, match="amplitudes"):
+        MatchTemplateManager.from_dict(bad)

@@ -88,3 +88,95 @@ def test_volume_rfft_matches_numpy():
     got = of.volume_to_rfft_fourier_slice(vol).numpy()
     ref = np.fft.fftshift(np.fft.rfftn(np.fft.fftshift(vol.numpy())), axes=(0, 1))
     assert np.allclose(got, ref, atol=1e-12)
+
+
+# ----------------------------------------------- magnification, bandpass, curve filters
+def _ctf(ang=30.0, pixel=1.0, mag=None, shape=(32, 32)):
+    return of.calculate_ctf_2d(torch.tensor([1.5]), 0.03, ang, 300.0, 2.7, 0.07, 0.0, pixel, shape,
+                               transform_matrix=mag)
+
+
+def test_ctf_identity_magnification_is_no_magnification():
+    assert torch.equal(_ctf(mag=torch.eye(2)), _ctf())
+
+
+def test_ctf_isotropic_magnification_is_a_pixel_size_change():
+    # A = s I scales every frequency by s: the CTF at pixel size p with A equals the CTF
+    # at pixel size p / s without
+    s = 1.02
+    got = _ctf(pixel=1.1, mag=s * torch.eye(2, dtype=torch.float64))
+    exp = _ctf(pixel=1.1 / s)
+    assert float((got - exp).abs().max()) < 1e-12
+
+
+def test_ctf_rotation_magnification_turns_the_astigmatism_axis():
+    # A = R(t) adds t to the polar angle of every frequency and keeps |k|: the same as
+    # an astigmatism angle turned by -t
+    t = math.radians(17.0)
+    r = torch.tensor([[math.cos(t), -math.sin(t)], [math.sin(t), math.cos(t)]], dtype=torch.float64)
+    got = _ctf(ang=30.0, mag=r)
+    exp = _ctf(ang=30.0 - 17.0)
+    assert float((got - exp).abs().max()) < 1e-12
+
+
+def _radial(shape):
+    h, w = shape
+    ky = np.fft.fftfreq(h)[:, None]
+    kx = np.fft.rfftfreq(w)[None, :]
+    return np.sqrt(ky**2 + kx**2)
+
+
+def test_bandpass_band_edges_and_midpoints():
+    shape = (64, 64)
+    f = _radial(shape)
+    lo, hi, fo = 0.1, 0.3, 0.05
+    b = of.bandpass_filter(lo, hi, fo, shape).numpy()
+    assert b.shape == (64, 33)
+    assert np.all(b[(f >= lo) & (f <= hi)] == 1.0)
+    assert np.all(b[(f > hi + fo) | (f < lo - fo)] == 0.0)
+    assert np.all((b >= 0) & (b <= 1))
+    # raised-cosine edges: 0.5 half way, monotone in the distance from the band
+    mid = of.bandpass_filter(lo, hi, 2 * abs(f[0, 20] - hi), shape).numpy()[0, 20]
+    assert abs(mid - 0.5) < 1e-12
+    edge = (f > hi) & (f <= hi + fo)
+    order = np.argsort(f[edge])
+    assert np.all(np.diff(b[edge][order]) <= 1e-15)
+
+
+def test_bandpass_limits():
+    shape = (32, 48)
+    assert torch.all(of.bandpass_filter(0.0, 1.0, 0.0, shape) == 1)  # the disabled defaults pass all
+    f = _radial(shape)
+    hard = of.bandpass_filter(0.05, 0.2, 0.0, shape).numpy()
+    assert np.array_equal(hard, ((f >= 0.05) & (f <= 0.2)).astype(float))
+    # the template-grid filter is radial: symmetric under ky -> -ky
+    b = of.bandpass_filter(0.05, 0.2, 0.03, (32, 32)).numpy()
+    assert np.array_equal(b[1:], b[1:][::-1])
+
+
+def test_arbitrary_curve_knots_segments_and_fill():
+    shape = (64, 64)
+    fq, am = [0.1, 0.2, 0.4], [1.0, 0.5, 2.0]
+    c = of.arbitrary_curve_filter(fq, am, shape).numpy()
+    f = _radial(shape)
+    assert np.all(c[f < 0.1] == 1.0) and np.all(c[f > 0.4] == 1.0)
+    seg = (f >= 0.2) & (f <= 0.4)
+    assert np.allclose(c[seg], 0.5 + (f[seg] - 0.2) / 0.2 * 1.5, atol=1e-12)
+    # knots: (0, 6) of a 64-wide grid has |k| = 6/64; (0, 0) row 0 column 8 -> 0.125
+    assert abs(c[0, 8] - (1.0 + (0.125 - 0.1) / 0.1 * -0.5)) < 1e-12
+
+
+def test_preprocess_with_bandpass_has_bandpass_dimensionality():
+    # after pre-processing the full Hermitian spectrum's energy equals the bandpass
+    # dimensionality sum(bp) + sum(bp[:, 1:-1])
+    g = torch.Generator().manual_seed(3)
+    img = torch.randn((64, 64), generator=g, dtype=torch.float64)
+    pre, wf = of.prepare_image_dft(img, (16, 9), bandpass=(0.05, 0.3, 0.05), curve=([0.1, 0.4], [1.0, 0.3]))
+    bp = of.bandpass_filter(0.05, 0.3, 0.05, (64, 64))
+    dim = float(bp.sum() + bp[:, 1:-1].sum())
+    energy = float((pre.abs() ** 2).sum() + (pre[:, 1:-1].abs() ** 2).sum())
+    assert abs(energy / dim - 1) < 1e-10
+    # all-pass settings reproduce the default preparation exactly
+    p0, w0 = of.prepare_image_dft(img, (16, 9))
+    p1, w1 = of.prepare_image_dft(img, (16, 9), bandpass=(0.0, 1.0, 0.0), curve=([0.0, 1.0], [1.0, 1.0]))
+    assert torch.allclose(p0, p1, atol=1e-13) and torch.allclose(w0, w1, atol=1e-13)

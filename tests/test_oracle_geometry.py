@@ -173,3 +173,82 @@ def test_combine_euler_angles_composition(seed):
     c = og.combine_euler_angles(a, b)
     assert torch.allclose(og.euler_to_rotmat_zyz(c), og.euler_to_rotmat_zyz(a) @ og.euler_to_rotmat_zyz(b),
                           atol=1e-10)
+
+
+# ------------------------------------------------------------ anisotropic magnification
+def _linear_slice(n, coef):
+    """Complex slice whose value is linear in the frequency: c0 + c1 x + c2 y (in samples)."""
+    fy = torch.arange(n, dtype=torch.float64) - n // 2
+    fx = torch.arange(n // 2 + 1, dtype=torch.float64)
+    gy, gx = torch.meshgrid(fy, fx, indexing="ij")
+    return coef[0] + coef[1] * gx + coef[2] * gy
+
+
+def test_transform_slice_identity_is_exact():
+    n = 16
+    vol = _rand_volume_rfft(n, 5)
+    s = og.extract_central_slices_rfft_3d(vol, og.euler_to_rotmat_zyz(torch.tensor([[10.0, 50.0, 200.0]])))
+    assert torch.equal(og.transform_slice_2d(s, torch.eye(2)), s)
+
+
+def test_transform_slice_quarter_turn_is_grid_permutation():
+    # A = [[0, -1], [1, 0]]: (x, y) -> (-y, x) lands on grid points; fold for x' < 0
+    n = 16
+    g = torch.Generator().manual_seed(6)
+    s = torch.complex(torch.randn(n, n // 2 + 1, generator=g, dtype=torch.float64),
+                      torch.randn(n, n // 2 + 1, generator=g, dtype=torch.float64))
+    t = og.transform_slice_2d(s, torch.tensor([[0.0, -1.0], [1.0, 0.0]]))
+    c = n // 2
+    for iy in range(n):
+        for kx in range(n // 2 + 1):
+            x, y = -(iy - c), kx
+            conj = x < 0
+            if conj:
+                x, y = -x, -y
+            inside = 0 <= y + c <= n - 1 and 0 <= x <= n // 2
+            exp = s[y + c, x] if inside else torch.zeros((), dtype=s.dtype)
+            assert torch.allclose(t[iy, kx], exp.conj() if conj else exp, atol=1e-14), (iy, kx)
+
+
+def test_transform_slice_reproduces_linear_functions():
+    # bilinear interpolation is exact on a linear function, so at every output point whose
+    # transformed frequency lies inside the grid the result is the function's value there
+    # (conjugated where the Hermitian fold applies); outside it is zero
+    n = 16
+    coef = (0.3 + 0.1j, -0.7 + 0.2j, 0.25 - 0.4j)
+    s = _linear_slice(n, coef)
+    a = torch.tensor([[1.03, 0.08], [-0.05, 0.97]], dtype=torch.float64)
+    t = og.transform_slice_2d(s, a)
+    c = n // 2
+    for iy in range(n):
+        for kx in range(n // 2 + 1):
+            k = a.numpy() @ np.array([kx, iy - c], dtype=np.float64)
+            conj = k[0] < 0
+            if conj:
+                k = -k
+            inside = 0 <= k[1] + c <= n - 1 and 0 <= k[0] <= n // 2
+            exp = coef[0] + coef[1] * k[0] + coef[2] * k[1] if inside else 0.0
+            if conj:
+                exp = np.conj(exp)
+            assert abs(complex(t[iy, kx]) - exp) < 1e-12, (iy, kx)
+
+
+def test_transform_slice_matches_scipy_bilinear():
+    n = 12
+    rng = np.random.default_rng(8)
+    s = rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+    a = np.array([[0.9, 0.2], [-0.15, 1.1]])
+    t = og.transform_slice_2d(torch.tensor(s), torch.tensor(a)).numpy()
+    for iy in range(n):
+        for kx in range(n // 2 + 1):
+            k = a @ np.array([kx / n, (iy - n // 2) / n])
+            conj = k[0] < 0
+            if conj:
+                k = -k
+            cyx = np.array([k[1] * n + n // 2, k[0] * n])
+            if np.all(cyx >= 0) and np.all(cyx <= np.array(s.shape) - 1):
+                val = ndimage.map_coordinates(s.real, cyx[:, None], order=1, mode="nearest")[0] + \
+                    1j * ndimage.map_coordinates(s.imag, cyx[:, None], order=1, mode="nearest")[0]
+            else:
+                val = 0.0
+            assert abs(t[iy, kx] - (np.conj(val) if conj else val)) < 1e-12
