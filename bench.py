@@ -413,6 +413,8 @@ def main():
                     help="run one device copy of this many MiB first (reference kernel for HBM byte counters)")
     ap.add_argument("--no-phase-timing", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time plain step launches instead of graph replays")
+    ap.add_argument("--option", action="append", default=[], metavar="KEY=VALUE",
+                    help="launch-shape option of the plan (tdtm_match_plan_set_option), repeatable")
     ap.add_argument("--fft-engine", type=int, default=0,
                     help="column transforms: 0 FP32 VALU; matrix cores 1 fp16 split, 2 bf16 split, "
                          "3 fp16, 4 bf16 (4096-row grids)")
@@ -495,6 +497,9 @@ def main():
     plan.set_option("graph", 0 if args.no_graph else 1)
     if args.fft_engine:
         plan.set_option("fft_engine", args.fft_engine)
+    for opt in args.option:
+        key, val = opt.split("=", 1)
+        plan.set_option(key, int(val))
     plan.set_inputs(img_dft, ctf, wf)
     plan.reset_stats()
     stream = torch.cuda.current_stream(dev)
@@ -647,6 +652,7 @@ def main():
                              "2.5/1.5 deg uniform grid"),
             "full_grid_orientations": full_grid,
             "full_grid_hours_extrapolated": full_grid * D / value / 3600.0,
+            **({"options": dict(o.split("=", 1) for o in args.option)} if args.option else {}),
         },
         "roofline": {
             "bound": "hbm",
