@@ -295,7 +295,8 @@ def bench_proposal(args) -> dict:
     dev = torch.device("cuda:0")
     out = proposal.run_experiment(dev, n_train=args.proposal_train, n_test=args.proposal_test,
                                   n_particles=args.proposal_particles, epochs=40, tile=args.proposal_tile,
-                                  bin_deg=args.proposal_bin_deg, operating_points=[(0.3, 0.9), (2.0, 0.9)],
+                                  bin_deg=args.proposal_bin_deg,
+                                  operating_points=[(0.3, 0.9), (2.0, 0.9), (0.0, 2.0)],
                                   log=lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True))
     ex_rate = out["orientations"] * out["test_micrographs"] / out["exhaustive_s"]
     return {
