@@ -206,7 +206,15 @@ __global__ __launch_bounds__(CP* HS / V) void k_proj_cols(const float2* __restri
   constexpr int T = HS / V;
   constexpr int WR = HS / 2 + 1;
   constexpr int LF = LdsSize<HS>::floats;
+  constexpr int SH = tw_shift(HS);
   __shared__ float lds[CP * 2 * LF];
+  // the transform's twiddles (table entries that are multiples of TW_N / HS) in LDS: the
+  // loop below then waits on global memory only for the next filter's values, not for
+  // twiddle loads queued behind the previous filter's stores
+  __shared__ float2 ltw[HS / 2];
+  for (int j = threadIdx.x; j < HS / 2; j += CP * T) ltw[j] = tw[j << SH];
+  __syncthreads();
+  const TwLds<SH> twl{ltw};
   const int c = threadIdx.x % CP, t = threadIdx.x / CP;
   const int kx = blockIdx.x * CP + c;
   const int b = blockIdx.y;
@@ -230,7 +238,7 @@ __global__ __launch_bounds__(CP* HS / V) void k_proj_cols(const float2* __restri
 #pragma unroll
       for (int i = 0; i < V; ++i) fv[i] = ok ? filt[(long)(f + 1) * HS * WR + (long)(t + i * T) * WR + kx] : 0.f;
     }
-    fft_wg<HS, V, true>(v, lds + c * 2 * LF, lds + c * 2 * LF + LF, t, tw);
+    fft_wg<HS, V, true>(v, lds + c * 2 * LF, lds + c * 2 * LF + LF, t, twl);
     if (ok) {
       float2* o = Tb + (long)(b + (long)nb * f) * HS * WR;
 #pragma unroll
@@ -260,7 +268,16 @@ __global__ __launch_bounds__(RPB*(HS / 2) / V) void k_proj_rows(const float2* __
   static_assert(T <= 64 && 64 % T == 0, "a row's threads lie in one wave");
   // every exchange below stays inside the row's threads, i.e. inside one wave
   float2* stg = reinterpret_cast<float2*>(lds + rl * 2 * LF);
-  for (int e = t; e < WR; e += T) stg[e] = row[e];
+  {
+    // all of the thread's loads first (one memory latency instead of one per element)
+    float2 ld[V];
+#pragma unroll
+    for (int i = 0; i < V; ++i) ld[i] = row[t + i * T];
+    const float2 nyq = t == 0 ? row[M] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int i = 0; i < V; ++i) stg[t + i * T] = ld[i];
+    if (t == 0) stg[M] = nyq;
+  }
   wave_lds_order();
   float2 v[V];
   const float2* g = stg;
@@ -386,6 +403,12 @@ __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __
   }
   const int y = y0 + r;
   const int k = blockIdx.y;
+  // twiddles e^{-2 pi i kx / WS} of the split below, loaded (after the transform) before
+  // any store is issued: a wait for a load also waits for the stores issued before it
+  constexpr int GW = WS / 4 + 1;                    // tile columns of a WS-point row (= G)
+  constexpr int GSTEP = RowFwdCfg<WS>::BLOCK / RR;  // tile columns per round of the block
+  constexpr int NGR = (GW + GSTEP - 1) / GSTEP;     // rounds
+  float2 wsp[NGR][CW];
   const float2 s = ae[k];
   const float2* prow = reinterpret_cast<const float2*>(P + ((long)k * hs + y) * hs);
   float2 v[V];
@@ -406,6 +429,13 @@ __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __
     fft_wg<M, V, false, NZJ>(v, lre, lim, t, tw);
   }
 #pragma unroll
+  for (int j = 0; j < NGR; ++j)
+#pragma unroll
+    for (int c = 0; c < CW; ++c) {
+      const int kx = CW * (threadIdx.x / RR + j * GSTEP) + c;
+      wsp[j][c] = kx <= M ? tw[kx * (TW_N / WS)] : make_float2(1.f, 0.f);
+    }
+#pragma unroll
   for (int i = 0; i < V; ++i) {
     int m = t + i * T;
     if constexpr (FOUR) m = fs_index<MF>(t, i);
@@ -421,19 +451,24 @@ __global__ __launch_bounds__(RowFwdCfg<WS>::BLOCK) void k_rowfwd(const float* __
   const int r2 = threadIdx.x % RR;
   const float* sre = lds + r2 * 2 * LF;
   const float* sim = sre + LF;
-  auto split = [&](int kx) {
+  auto split = [&](int kx, float2 wk) {
     if (kx > M) return make_float2(0.f, 0.f);
     const int a = kx & (M - 1), bi = (M - kx) & (M - 1);
     const float2 A = make_float2(sre[lds_pad(a)], sim[lds_pad(a)]);
     const float2 Bc = make_float2(sre[lds_pad(bi)], -sim[lds_pad(bi)]);
     const float2 E = c_scale(c_add(A, Bc), 0.5f);
     const float2 O = c_scale(c_sub(A, Bc), 0.5f);
-    const float2 wo = c_mul(tw[kx * (TW_N / WS)], O);  // e^{-2 pi i kx/WS} * O
-    return make_float2(E.x + wo.y, E.y - wo.x);         // E - i*wo
+    const float2 wo = c_mul(wk, O);                // e^{-2 pi i kx/WS} * O
+    return make_float2(E.x + wo.y, E.y - wo.x);    // E - i*wo
   };
-  for (int g = threadIdx.x / RR; g < G; g += RowFwdCfg<WS>::BLOCK / RR) {
-    const float2 X0 = split(CW * g), X1 = split(CW * g + 1);
-    *reinterpret_cast<float4*>(out + ((long)g * hs + y0 + r2) * CW) = make_float4(X0.x, X0.y, X1.x, X1.y);
+  (void)G;  // G == GW for every plan (tile columns of the row length)
+#pragma unroll
+  for (int j = 0; j < NGR; ++j) {
+    const int g = threadIdx.x / RR + j * GSTEP;
+    if (g < GW) {
+      const float2 X0 = split(CW * g, wsp[j][0]), X1 = split(CW * g + 1, wsp[j][1]);
+      *reinterpret_cast<float4*>(out + ((long)g * hs + y0 + r2) * CW) = make_float4(X0.x, X0.y, X1.x, X1.y);
+    }
   }
 }
 

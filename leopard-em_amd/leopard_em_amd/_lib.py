@@ -74,11 +74,12 @@ class TdtmError(RuntimeError):
 
 
 def load(path: str | None = None):
-    """Load (once) and return the ctypes library handle; raises if it is not built."""
+    """Load (once) and return the ctypes library handle; raises if it is not built.
+    ``TDTM_LIBRARY`` in the environment names another build of the library to load."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("TDTM_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise TdtmError(f"tdtm HIP library not built: {p} is missing (run __graft_entry__.build())")
     lib = ctypes.CDLL(p)
