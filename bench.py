@@ -218,7 +218,8 @@ def bench_refine(args) -> dict:
 
     def run(sl):
         return refine_template_ragged(stack[sl], tdft, poses[sl], offs[sl], dfos[sl], du[sl], dv[sl], da[sl],
-                                      pxos[sl], mean[sl], std[sl], kw, filt[sl], dev, batch_size=32)
+                                      pxos[sl], mean[sl], std[sl], kw, filt[sl], dev,
+                                      batch_size=args.refine_batch)
 
     run(slice(0, 1))  # warm-up (plan, graph capture)
     torch.cuda.synchronize(dev)
@@ -427,6 +428,7 @@ def main():
                     help="refine: particle refinement throughput; proposal: learned orientation proposals")
     ap.add_argument("--refine-particles", type=int, default=24,
                     help="crops timed by --workload refine (2000 = the whole 8-micrograph job)")
+    ap.add_argument("--refine-batch", type=int, default=32, help="orientation offsets per refinement batch")
     ap.add_argument("--proposal-train", type=int, default=64, help="training micrographs of --workload proposal")
     ap.add_argument("--proposal-test", type=int, default=16, help="held-out micrographs of --workload proposal")
     ap.add_argument("--proposal-particles", type=int, default=6, help="particles per proposal micrograph")
