@@ -81,11 +81,11 @@ int tdtm_match_plan_set_magnification(tdtm_match_plan_t plan, const double* mag_
  * per column group, 0 = automatic), "colpass_twiddles" (0: global table, 1: LDS copy
  * for the four-step column transforms where it fits, 2: each lane's twiddles built once
  * in registers for all its correlations, default; one correlation per iteration,
- * otherwise the LDS copy), "colpass_store" (0: direct 8-byte stores; 1: rows of both
- * columns through LDS, 16-byte full-tile stores, default), "colpass_persistent" (0: one
+ * otherwise the LDS copy), "colpass_store" (0: direct 8-byte stores; 1: rows
+ * through LDS, 16-byte stores of two rows of one column of a tile, default), "colpass_persistent" (0: one
  * block per tile column and correlation chunk; 1: as many blocks as are resident at once,
- * each pair of blocks taking an equal share of the (tile-column pair, correlation) items,
- * default; two-column blocks with one correlation per iteration only), "rowpass_staging" (0: synchronous de-interleaved,
+ * each group of blocks writing the same lines taking an equal share of the (tile-column
+ * pair, correlation) items, default; one correlation per iteration only), "rowpass_staging" (0: synchronous de-interleaved,
  * 1: asynchronous raw double buffer, 2: asynchronous raw copy + de-interleaving reformat,
  * 3: synchronous de-interleaved with batched loads; default 3), "rowpass_twiddles" (0: global
  * table, 1: LDS copy, default; needs staging 0 or 3), "rowpass_rows" (rows of a 4-row tile

@@ -292,14 +292,16 @@ __global__ __launch_bounds__(MF_BLOCK) void k_colpass_mfma(const float2* __restr
         for (int r = 0; r < 4; ++r) xch[c][mf_swz(mf_pos(j, q, 4 * g + r))] = make_float2(dre[r], -dim[r]);
       }
       __syncthreads();
-      // whole tile rows (both columns, 16 bytes) to the intermediate
-      float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * MF_CW) + (long)gg * MF_TR;
+      // 16-byte pieces (rows 2h, 2h+1 of one column) of the column-major tiles of the
+      // intermediate: 4 consecutive threads fill a tile
+      float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * MF_CW) + (long)gg * (MF_TR * MF_CW / 2);
 #pragma unroll
       for (int jj = 0; jj < HH / MF_BLOCK; ++jj) {
-        const int y = tid + jj * MF_BLOCK;
-        const int ys = mf_swz(y);
-        const float2 a = xch[0][ys], b = xch[1][ys];
-        gout[((long)(y / MF_TR) * Gs) * MF_TR + (y % MF_TR)] = make_float4(a.x, a.y, b.x, b.y);
+        const int u = tid + jj * MF_BLOCK;  // piece: tile row u / 4, column (u / 2) % 2, rows 2 (u % 2) + 0, 1
+        const int trow = u / (MF_TR / 2 * MF_CW), col = (u / (MF_TR / 2)) % MF_CW, h2 = u % (MF_TR / 2);
+        const int y = trow * MF_TR + 2 * h2;
+        const float2 a = xch[col][mf_swz(y)], b = xch[col][mf_swz(y + 1)];
+        gout[(long)trow * Gs * (MF_TR * MF_CW / 2) + (col * MF_TR + 2 * h2) / 2] = make_float4(a.x, a.y, b.x, b.y);
       }
       // the next F1 writes the exchange arrays: every thread's tile reads come first
       __syncthreads();

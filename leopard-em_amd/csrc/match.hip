@@ -36,6 +36,12 @@ namespace tdtm {
 constexpr int V = 8;   // values per thread in every FFT
 constexpr int CW = 2;  // columns per column-pass block (tile width)
 constexpr int TR = 4;  // rows per tile (= rows per row-pass block)
+// Position of (row r, column c) inside a tile of the intermediate, in complex elements: the
+// tile is column-major, each column's TR rows contiguous (32 bytes), so that a column-pass
+// block of either width writes whole 16-byte pieces (two rows of one column).
+__host__ __device__ constexpr int tile_off(int r, int c) { return c * TR + r; }
+// float4 index of the piece (column c, rows 2 h2, 2 h2 + 1) inside a tile
+__host__ __device__ constexpr int tile_f4(int c, int h2) { return (c * TR + 2 * h2) / 2; }
 static_assert(MF_CW == CW && MF_TR == TR, "the matrix-core column pass writes the same tiles");
 constexpr int TDTM_LDS_BYTES = 160 * 1024;  // LDS one workgroup may allocate on gfx950
 constexpr int REFINE_CHUNKS_MAX = 64;        // correlation chunks of a refinement row pass
@@ -507,7 +513,7 @@ __device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // are followed by the next forward E1 stores to the same addresses by the same thread),
 // and xs needs no barrier after its reads: the next writes to xs come after the next
 // iteration's two E1 barriers, which every wave reaches only after its reads.
-template <int HH, int NZJ, int NT, int NX, bool FULL, bool TS, class TWT>
+template <int HH, int NZJ, int NT, int NX, bool FULL, bool TS, int NCOL = CW, class TWT>
 __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V], const float2* __restrict__ Rb,
                                              int hs, int G, int Gs, int g, int c, int t, int k, int klast, int ystore,
                                              bool dc, const float2 (&xi)[V], float* const (&re)[NT],
@@ -533,18 +539,28 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
   fs_perm2nat_n<HH, true, NT, true, false>(v, re, im, t, tw);
   if constexpr (TS) {
     static_assert(NT == 1, "transposed stores take one correlation per iteration");
-    // column c's rows into plane c (lane-consecutive rows: conflict free), then each
-    // thread takes 4 rows, both columns
+    static_assert(NCOL == 1 || NCOL == CW, "columns per block");
+    // the block's columns' rows into their planes (lane-consecutive rows: conflict free),
+    // then 16-byte pieces -- rows 2h, 2h+1 of one column of a tile -- to the intermediate:
+    // consecutive threads take consecutive pieces of a tile row, so a tile is filled by 4
+    // lanes (2 with one column per block, the other column's block filling the rest)
+    const int cp = NCOL == 1 ? 0 : c;  // plane of this thread's column
 #pragma unroll
-    for (int i = 0; i < V; ++i) xs[c * HH + t + i * T] = v[0][i];
+    for (int i = 0; i < V; ++i) xs[cp * HH + t + i * T] = v[0][i];
     lds_barrier();
-    const int q = c * T + t;  // thread index in the block (2 T threads)
-    float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * CW) + (long)g * TR;
+    const int q = cp * T + t;  // thread index in the block (NCOL T threads)
+    constexpr int PPT = TR / 2 * NCOL;  // pieces per tile written by this block
+    float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * CW) + (long)g * (TR * CW / 2);
 #pragma unroll
     for (int j = 0; j < HH / (2 * T); ++j) {
-      const int y = q + j * 2 * T;
-      const float2 a = xs[y], b = xs[HH + y];
-      if (FULL || y < ystore) gout[((long)(y / TR) * Gs) * TR + (y % TR)] = make_float4(a.x, a.y, b.x, b.y);
+      const int u = q + j * NCOL * T;  // piece index: tile row u / PPT, piece u % PPT
+      const int trow = u / PPT;
+      const int col = NCOL == 1 ? c : (u % PPT) / (TR / 2);
+      const int h2 = u % (TR / 2);
+      const int y = trow * TR + 2 * h2;
+      const float2* src = xs + (NCOL == 1 ? 0 : col) * HH + y;
+      const float2 a = src[0], b = src[1];
+      if (FULL || y < ystore) gout[(long)trow * Gs * (TR * CW / 2) + tile_f4(col, h2)] = make_float4(a.x, a.y, b.x, b.y);
     }
   } else {
     // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
@@ -552,7 +568,7 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
     constexpr long ISTRIDE = (long)(T / TR) * TR * CW;  // complex per step of i, per tile column
 #pragma unroll
     for (int n = 0; n < NT; ++n) {
-      float2* gout = Gb + (long)(k + n) * HH * Gs * CW + ((long)(t / TR) * Gs + g) * TR * CW + (t % TR) * CW + c;
+      float2* gout = Gb + (long)(k + n) * HH * Gs * CW + ((long)(t / TR) * Gs + g) * TR * CW + tile_off(t % TR, c);
 #pragma unroll
       for (int i = 0; i < V; ++i)
         if (FULL || t + i * T < ystore) gout[(long)i * ISTRIDE * Gs] = v[n][i];
@@ -567,8 +583,10 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // tile).  TWM = 1 reads the twiddles from an LDS copy of the table entries the
 // four-step transforms of size HH use; TWM = 2 builds each lane's twiddles once, in
 // registers, for all the block's correlations; FULL = every row is stored (ystore == HH).
+// (launch bounds: one-column blocks at least 4 waves per SIMD, i.e. two blocks per CU and
+// at most 128 VGPRs)
 template <int HH, int NZJ, int CWB, int NT, int TWM, bool FULL, bool TS = false>
-__global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
+__global__ __launch_bounds__(CWB*(HH / V), CWB == 1 ? 4 : 1) void k_colpass(const float2* __restrict__ Rb, const float2* __restrict__ Xt,
                                                          int hs, int G, int Gs, int K, int kchunk, int ystore,
                                                          const float2* __restrict__ tw_global, float2* __restrict__ Gb,
                                                          int persist) {
@@ -582,18 +600,19 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   static_assert(TWM == 0 || FOUR, "the LDS twiddle table serves the four-step path");
   // TWM == 1: lane-ordered copy of the four-step twiddle bases (FsTwLayout) after the
   // transforms' work areas
-  // TS: the transposed-store planes (2 HH float2) after both
+  // TS: the transposed-store planes (one of HH float2 per column) after both
   constexpr int XS0 = CWB * NT * 2 * LF + (TWM == 1 ? FsTwLayout<HF>::floats : 0);
-  __shared__ float lds[XS0 + (TS ? 4 * HH : 0)];
+  __shared__ float lds[XS0 + (TS ? 2 * CWB * HH : 0)];
   // block -> (tile column g, column c): the blocks that write the same 128-byte lines of
   // Gb -- tile columns 2m and 2m+1 (and both columns of each with one column per block)
   // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
   // robin) and complete the lines in its L2; the grid is padded to whole groups and the
   // blocks past the last tile column exit
-  // Persistent launch (persist > 0; two-column blocks, one correlation per iteration):
-  // one block per resident slot; the blocks b and b + 8 (one XCD) form a pair that takes
-  // tile columns 2m and 2m + 1 of the same work items (m, k), and pair j takes the items
-  // [j persist, (j + 1) persist) of the m-major list of ceil(G/2) x K items -- every pair
+  // Persistent launch (persist > 0; one correlation per iteration): one block per resident
+  // slot; the blocks that write the same lines (b and b + 8 with two columns per block; b,
+  // b + 8, b + 16, b + 24 with one) form a group on one XCD that takes tile columns 2m and
+  // 2m + 1 (both columns) of the same work items (m, k), and group j takes the items
+  // [j persist, (j + 1) persist) of the m-major list of ceil(G/2) x K items -- every group
   // gets the same number of correlations, the image column is reloaded where m changes.
   int g = 0, c = 0;
   {
@@ -625,7 +644,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
   if constexpr (CWB == CW) c = cl;
   float* base = lds + cl * NT * 2 * LF;
   float2* lds2 = reinterpret_cast<float2*>(lds + XS0);
-  static_assert(!TS || (CWB == CW && NT == 1 && FOUR), "transposed stores: layout");
+  static_assert(!TS || (NT == 1 && FOUR), "transposed stores: layout");
   float2 xi[V];
   // image column g (permuted order on the four-step path)
   auto load_image = [&](int gg) {
@@ -636,7 +655,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       xi[i] = Xt[((long)gg * HH + ky) * CW + c];
     }
   };
-  if constexpr (FOUR && NT == 1 && CWB == CW) {
+  if constexpr (FOUR && NT == 1) {
     // correlations k0 .. k1-1 of tile column gg, the first iteration peeled so that the
     // loop is always entered with the prefetched inputs followed by one iteration's
     // stores in the memory queue (the wait for the inputs then leaves the stores in flight)
@@ -649,15 +668,17 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       const int klast = k1 - 1;
       int k = k0;
       colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, gg, c, t, k, klast);
-      colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb,
-                                            lds2);
+      colpass_four<HF, NZJ, 1, 1, FULL, TS, CWB>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1,
+                                                 tw, Gb, lds2);
       for (++k; k < k1; ++k)
-        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1, tw,
-                                              Gb, lds2);
+        colpass_four<HF, NZJ, 1, 1, FULL, TS, CWB>(in1, Rb, hs, G, Gs, gg, c, t, k, klast, ystore, dc, xi, re1, im1,
+                                                   tw, Gb, lds2);
     };
     if (persist > 0) {
       const int b = blockIdx.x, q = b >> 3;
-      const int half = q & 1, pair = (q >> 1) * 8 + (b & 7);
+      // group of the blocks that share lines; half = tile column 2m + half of the group
+      const int half = CWB == CW ? (q & 1) : ((q >> 1) & 1);
+      const int pair = CWB == CW ? (q >> 1) * 8 + (b & 7) : (q >> 2) * 8 + (b & 7);
       const long total = (long)((G + 1) / 2) * K;
       long item = (long)pair * persist;
       const long iend = min(total, item + persist);
@@ -693,26 +714,26 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
       // wait for the inputs then leaves the stores in flight)
       if (k + 1 < k1) {
         colpass_load<HF, NZJ, 2>(in, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 2, 2, FULL, TS>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
+        colpass_four<HF, NZJ, 2, 2, FULL, TS, CWB>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
         k += 2;
       }
       for (; k + 1 < k1; k += 2)
-        colpass_four<HF, NZJ, 2, 2, FULL, TS>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
+        colpass_four<HF, NZJ, 2, 2, FULL, TS, CWB>(in, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re2, im2, tw, Gb, lds2);
       // a single last correlation: its input was prefetched into in[0], unless the chunk
       // holds only this one
       if (k < k1) {
         if (k == k0) colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 0, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
+        colpass_four<HF, NZJ, 1, 0, FULL, TS, CWB>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
       }
     } else {
       const int klast = k1 - 1;
       if (k < k1) {  // peeled first iteration (see the paired loop)
         colpass_load<HF, NZJ, 1>(in1, Rb, hs, G, g, c, t, k, klast);
-        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
+        colpass_four<HF, NZJ, 1, 1, FULL, TS, CWB>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
         ++k;
       }
       for (; k < k1; ++k)
-        colpass_four<HF, NZJ, 1, 1, FULL, TS>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
+        colpass_four<HF, NZJ, 1, 1, FULL, TS, CWB>(in1, Rb, hs, G, Gs, g, c, t, k, klast, ystore, dc, xi, re1, im1, tw, Gb, lds2);
     }
   } else {
     float* lre = base;
@@ -734,7 +755,7 @@ __global__ __launch_bounds__(CWB*(HH / V)) void k_colpass(const float2* __restri
 #pragma unroll
       for (int i = 0; i < V; ++i) {
         const int yy = t + i * T;
-        if (FULL || yy < ystore) gout[(((long)(yy / TR) * Gs + g) * TR + (yy % TR)) * CW + c] = v[i];
+        if (FULL || yy < ystore) gout[((long)(yy / TR) * Gs + g) * TR * CW + tile_off(yy % TR, c)] = v[i];
       }
     }
   }
@@ -799,27 +820,32 @@ __device__ __forceinline__ void tile_block_async(const float2* __restrict__ src,
 // c2r from the staged tile (imaginary parts of DC / Nyquist dropped), run the inverse
 // transform in the same buffer, and hand the natural-order result to `epi(k, v)`:
 // thread (row r, t) holds v[i] = (x[2n], x[2n+1]) for n = t + i*T of row y = p*TR + r.
-// Synchronous tile staging: 16-byte loads (two columns of one row), stored de-interleaved
-// into the per-row split re / im regions at stg_pad positions.
-// `src` points at the block's first row inside the tile row; a 16-byte chunk is one row's
-// two columns of one tile, chunks of consecutive tiles are TR chunks apart.
+// Synchronous tile staging: 8-byte loads (one row of one column of a tile), stored
+// de-interleaved into the per-row split re / im regions at stg_pad positions.
+// `src` points at the block's first row inside the tile row's first tile; element (row
+// rr, column c) of tile g is at src[g TR CW + tile_off(rr, c)].  Element e of the block
+// is (tile g, column c, row rr) with rr fastest, so consecutive threads read consecutive
+// addresses of a column's rows.
+template <int WS, int RB>
+__device__ __forceinline__ void tile_stage_one(float* dst, int e, float2 x) {
+  using C = RowCfg<WS, RB>;
+  const int rr = e % RB, c = (e / RB) % CW, g = e / (RB * CW);
+  const int a = stg_pad(CW * g + c);
+  float* re = dst + rr * C::RSTR;
+  re[a] = x.x;
+  re[a + C::LF] = x.y;
+}
+
+template <int WS, int RB>
+__device__ __forceinline__ const float2& tile_src(const float2* __restrict__ src, int e) {
+  const int rr = e % RB, c = (e / RB) % CW, g = e / (RB * CW);
+  return src[g * TR * CW + tile_off(rr, c)];
+}
+
 template <int WS, int RB>
 __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, float* dst) {
   using C = RowCfg<WS, RB>;
-  constexpr int NF4 = C::TILE / 2;
-  constexpr int BLOCK = C::BLOCK;
-  const float4* s4 = reinterpret_cast<const float4*>(src);
-  for (int f = threadIdx.x; f < NF4; f += BLOCK) {
-    const int g = f / RB, rr = f % RB;
-    const float4 x = s4[g * TR + rr];
-    const int a = stg_pad(2 * g);
-    float* re = dst + rr * C::RSTR;
-    float* im = re + C::LF;
-    re[a] = x.x;
-    re[a + 1] = x.z;
-    im[a] = x.y;
-    im[a + 1] = x.w;
-  }
+  for (int e = threadIdx.x; e < C::TILE; e += C::BLOCK) tile_stage_one<WS, RB>(dst, e, tile_src<WS, RB>(src, e));
 }
 
 // Synchronous tile staging with every load of the thread issued before the first LDS
@@ -827,55 +853,47 @@ __device__ __forceinline__ void tile_block_sync(const float2* __restrict__ src, 
 template <int WS, int RB>
 __device__ __forceinline__ void tile_block_sync_batched(const float2* __restrict__ src, float* dst) {
   using C = RowCfg<WS, RB>;
-  constexpr int NF4 = C::TILE / 2;
+  constexpr int NE = C::TILE;
   constexpr int BLOCK = C::BLOCK;
-  constexpr int ROUNDS = (NF4 + BLOCK - 1) / BLOCK;
-  static_assert((ROUNDS - 1) * BLOCK < NF4 && ROUNDS * BLOCK >= NF4, "only the last round is partial");
-  const float4* s4 = reinterpret_cast<const float4*>(src);
-  float4 x[ROUNDS];
+  constexpr int ROUNDS = (NE + BLOCK - 1) / BLOCK;
+  static_assert((ROUNDS - 1) * BLOCK < NE && ROUNDS * BLOCK >= NE, "only the last round is partial");
+  float2 x[ROUNDS];
 #pragma unroll
   for (int j = 0; j < ROUNDS; ++j) {
-    const int f = threadIdx.x + j * BLOCK;
-    if (j + 1 < ROUNDS || f < NF4) x[j] = s4[(f / RB) * TR + f % RB];
+    const int e = threadIdx.x + j * BLOCK;
+    if (j + 1 < ROUNDS || e < NE) x[j] = tile_src<WS, RB>(src, e);
   }
 #pragma unroll
   for (int j = 0; j < ROUNDS; ++j) {
-    const int f = threadIdx.x + j * BLOCK;
-    if (j + 1 < ROUNDS || f < NF4) {
-      const int g = f / RB, rr = f % RB;
-      const int a = stg_pad(2 * g);
-      float* re = dst + rr * C::RSTR;
-      float* im = re + C::LF;
-      re[a] = x[j].x;
-      re[a + 1] = x[j].z;
-      im[a] = x[j].y;
-      im[a + 1] = x[j].w;
-    }
+    const int e = threadIdx.x + j * BLOCK;
+    if (j + 1 < ROUNDS || e < NE) tile_stage_one<WS, RB>(dst, e, x[j]);
   }
 }
 
-// Reformat of a raw staged tile block (HBM layout) into the de-interleaved per-row
-// re / im regions: consecutive 16-byte reads, paired 8-byte writes.
+// Reformat of a raw staged tile block (HBM layout: whole tiles, TR rows) into the
+// de-interleaved per-row regions: consecutive 16-byte reads (rows 2h, 2h+1 of one column
+// of a tile), 4-byte writes into two rows.
 template <int WS>
 __device__ __forceinline__ void tile_block_reformat(const float* raw, float* dst) {
   using C = RowCfg<WS>;
   constexpr int NF4 = C::TILE / 2;
   constexpr int BLOCK = TR * C::T;
   constexpr int ROUNDS = (NF4 + BLOCK - 1) / BLOCK;
+  constexpr int P4 = TR * CW / 2;  // float4 per tile
   const float4* s4 = reinterpret_cast<const float4*>(raw);
 #pragma unroll
   for (int j = 0; j < ROUNDS; ++j) {
     const int f = threadIdx.x + j * BLOCK;
     if (f < NF4) {
       const float4 x = s4[f];
-      const int g = f / TR, rr = f % TR;
-      const int a = stg_pad(2 * g);
-      float* re = dst + rr * C::RSTR;
-      float* im = re + C::LF;
-      re[a] = x.x;
-      re[a + 1] = x.z;
-      im[a] = x.y;
-      im[a + 1] = x.w;
+      const int g = f / P4, c = (f % P4) / (TR / 2), r = 2 * (f % (TR / 2));
+      const int a = stg_pad(CW * g + c);
+      float* re0 = dst + r * C::RSTR;
+      float* re1 = re0 + C::RSTR;
+      re0[a] = x.x;
+      re0[a + C::LF] = x.y;
+      re1[a] = x.z;
+      re1[a + C::LF] = x.w;
     }
   }
 }
@@ -954,7 +972,7 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       return tw_global;
   }();
   const long kstride = (long)(HH / TR) * C::TILEF;
-  const float2* gsrc = Gb + (long)p * C::TILEF + r0 * CW;
+  const float2* gsrc = Gb + (long)p * C::TILEF + tile_off(r0, 0);
   float* raw2 = lds;                  // MODE 2: raw tile
   float* work2 = lds + 2 * C::TILE;   // MODE 2: de-interleaved regions / transform work area
   const int K = kend;
@@ -999,8 +1017,8 @@ __device__ __forceinline__ void rowpass_loop(const float2* __restrict__ Gb, int 
       const int bm = M - mm;
       float2 A, Bc;
       if (ASYNC) {  // raw copy of the HBM tile layout
-        A = stg[((mm >> 1) * TR + r) * CW + (mm & 1)];
-        Bc = stg[((bm >> 1) * TR + r) * CW + (bm & 1)];
+        A = stg[(mm >> 1) * TR * CW + tile_off(r, mm & 1)];
+        Bc = stg[(bm >> 1) * TR * CW + tile_off(r, bm & 1)];
       } else {
         A = make_float2(sre[stg_pad(mm)], sim[stg_pad(mm)]);
         Bc = make_float2(sre[stg_pad(bm)], sim[stg_pad(bm)]);
@@ -1660,7 +1678,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   Kern kern = nullptr;
 #define TDTM_COLPASS_T(CWB_, NT_, TWM_, FULL_)                                                       \
   do {                                                                                             \
-    if constexpr (CWB_ == CW && NT_ == 1 && FOUR) {                                                \
+    if constexpr (NT_ == 1 && FOUR) {                                                              \
       if (p->opt_colpass_store == 1) kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_, true>;       \
       else kern = k_colpass<HH, NZJ, CWB_, NT_, TWM_, FULL_, false>;                              \
     } else {                                                                                       \
@@ -1669,7 +1687,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   } while (0)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
-    constexpr int xs_ = (CWB_ == CW && NT_ == 1) ? 4 * HH : 0; /* transposed-store planes */ \
+    constexpr int xs_ = NT_ == 1 ? 2 * CWB_ * HH : 0; /* transposed-store planes */          \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF + xs_) * 4 <= TDTM_LDS_BYTES;   \
     constexpr bool regs_ = FOUR && NT_ == 1;                                              \
     if (rtw && regs_) {                                                                   \
@@ -1705,14 +1723,18 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
     }
     return p->occ_per_cu;
   };
-  if (FOUR && cols == CW && nt == 1 && p->opt_colpass_persistent) {
-    // persistent schedule: as many blocks as are resident at once (a multiple of 16,
-    // pairs b, b + 8), each pair taking an equal share of the ceil(G/2) x K items
+  if (FOUR && nt == 1 && p->opt_colpass_persistent) {
+    // persistent schedule: as many blocks as are resident at once, in whole XCD groups of
+    // the blocks that share lines (b, b + 8 with two columns per block; b .. b + 24 with
+    // one: multiples of 16 / 32 blocks), each group taking an equal share of the
+    // ceil(G/2) x K items
     const int per_cu = blocks_per_cu();
-    long nblk = ((long)per_cu * p->num_cu) / 16 * 16;
-    if (nblk < 16) nblk = 16;
+    const long gsize = cols == CW ? 2 : 4;  // blocks per group
+    const long unit = 8 * gsize;
+    long nblk = ((long)per_cu * p->num_cu) / unit * unit;
+    if (nblk < unit) nblk = unit;
     const long total = (long)((p->G + 1) / 2) * K;
-    const long npairs = nblk / 2;
+    const long npairs = nblk / gsize;
     const int persist = (int)((total + npairs - 1) / npairs);
     hipLaunchKernelGGL(kern, dim3((unsigned)nblk), dim3(threads), 0, st, (const float2*)p->Rb,
                        (const float2*)p->Xt, p->h, p->G, p->Gs, K, K, ystore, (const float2*)p->tw, p->Gb, persist);

@@ -50,23 +50,73 @@ def test_async_tile_copy_covers_every_chunk_once(WS):
     assert sorted(copied) == list(range(NF4))
 
 
-@pytest.mark.parametrize("WS", [64, 128, 256, 512, 1024, 2048, 4096])
+def _tile_off(r, c):
+    """tile_off: column-major 4-row x 2-column tiles of the intermediate."""
+    return c * TR + r
+
+
+@pytest.mark.parametrize("WS", [64, 128, 256, 512, 1024, 2048, 4096, 8192])
 def test_batched_sync_staging_rounds(WS):
-    """tile_block_sync_batched: ROUNDS = ceil(NF4 / BLOCK) loads per thread, only the last
-    round partial; every chunk (tile g, row rr) loaded once, at source offset g*TR + rr."""
+    """tile_block_sync_batched: ROUNDS = ceil(NE / BLOCK) 8-byte loads per thread, only the
+    last round partial; every element (tile g, column c, row rr) loaded once, from
+    g*TR*CW + tile_off(rr, c) of the tile row, and written once, to row rr's position 2g+c."""
     for RB in (1, 2, 4):
         M = WS // 2
         G = (M + 1 + CW - 1) // CW
-        NF4 = G * RB * CW // 2
+        NE = G * RB * CW
         BLOCK = RB * (M // V)
-        ROUNDS = (NF4 + BLOCK - 1) // BLOCK
-        assert (ROUNDS - 1) * BLOCK < NF4 <= ROUNDS * BLOCK
-        src = []
+        ROUNDS = (NE + BLOCK - 1) // BLOCK
+        assert (ROUNDS - 1) * BLOCK < NE <= ROUNDS * BLOCK
+        src, dst = [], []
         for tid in range(BLOCK):
             for j in range(ROUNDS):
-                f = tid + j * BLOCK
-                if j + 1 < ROUNDS or f < NF4:
-                    src.append((f // RB) * TR + f % RB)
-        assert len(src) == NF4 == len(set(src))
-        # all chunks lie inside the tile row's G tiles of TR chunks
-        assert max(src) < G * TR
+                e = tid + j * BLOCK
+                if j + 1 < ROUNDS or e < NE:
+                    rr, c, g = e % RB, (e // RB) % CW, e // (RB * CW)
+                    src.append(g * TR * CW + _tile_off(rr, c))
+                    dst.append((rr, CW * g + c))
+        assert len(src) == NE == len(set(src)) == len(set(dst))
+        # within the tile row's G tiles, and only rows 0 .. RB-1 of each column
+        assert max(src) < G * TR * CW and all(s % TR < RB for s in src)
+        assert sorted(dst) == [(r, x) for r in range(RB) for x in range(CW * G)]
+
+
+@pytest.mark.parametrize("HH", [512, 1024, 2048, 4096])
+@pytest.mark.parametrize("NCOL", [1, 2])
+def test_column_pass_pieces_cover_tiles_once(HH, NCOL):
+    """colpass_four's transposed stores: thread q (NCOL * T threads) writes pieces
+    u = q + j NCOL T, j < HH / (2 T): tile row u / PPT, column (NCOL == 1: the block's
+    column c), rows 2 h2, 2 h2 + 1 -- together every (row, column) of the block's columns
+    once, 16-byte aligned, and 4 (2 with one column) consecutive threads per tile."""
+    T = HH // V
+    PPT = TR // 2 * NCOL
+    for c_block in range(CW if NCOL == 1 else 1):
+        seen = []
+        for q in range(NCOL * T):
+            for j in range(HH // (2 * T)):
+                u = q + j * NCOL * T
+                trow = u // PPT
+                col = c_block if NCOL == 1 else (u % PPT) // (TR // 2)
+                h2 = u % (TR // 2)
+                y = trow * TR + 2 * h2
+                f4 = trow * 1000 * (TR * CW // 2) + (col * TR + 2 * h2) // 2  # tile row stride (any Gs)
+                assert (col * TR + 2 * h2) % 2 == 0
+                seen += [(y, col), (y + 1, col)]
+                assert _tile_off(y % TR, col) // 2 == (col * TR + 2 * h2) // 2
+                assert f4 >= 0
+        cols = [c_block] if NCOL == 1 else list(range(CW))
+        assert sorted(seen) == sorted((y, c) for y in range(HH) for c in cols)
+
+
+def test_mfma_pieces_cover_tiles_once():
+    """k_colpass_mfma's stores (MF_BLOCK = 1024 threads, HH = 4096): every (row, column) of
+    the tile column once."""
+    HH, BLOCK = 4096, 1024
+    seen = []
+    for tid in range(BLOCK):
+        for jj in range(HH // BLOCK):
+            u = tid + jj * BLOCK
+            trow, col, h2 = u // (TR // 2 * CW), (u // (TR // 2)) % CW, u % (TR // 2)
+            y = trow * TR + 2 * h2
+            seen += [(y, col), (y + 1, col)]
+    assert sorted(seen) == sorted((y, c) for y in range(HH) for c in range(CW))
