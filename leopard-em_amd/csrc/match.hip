@@ -507,7 +507,7 @@ __device__ __forceinline__ void colpass_load(float2 (&in)[NT][NZJ < V ? NZJ : V]
 // TS (two columns per block, one correlation per iteration): the result goes through LDS
 // so that every thread stores whole rows (both columns, 16 bytes) and 4 consecutive lanes
 // fill one 64-byte tile -- full-tile stores instead of 8-byte stores into tile halves.
-// xs = 2 * HH float2 of LDS apart from the transform areas.
+// xs = NCOL * HH float2 of LDS apart from the transform areas (16-byte aligned).
 // Barriers per iteration: the forward and inverse E1 exchanges and one before the
 // transposed reads.  The inverse transform's closing barrier is left out (its E1 loads
 // are followed by the next forward E1 stores to the same addresses by the same thread),
@@ -540,17 +540,28 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
   if constexpr (TS) {
     static_assert(NT == 1, "transposed stores take one correlation per iteration");
     static_assert(NCOL == 1 || NCOL == CW, "columns per block");
-    // the block's columns' rows into their planes (lane-consecutive rows: conflict free),
-    // then 16-byte pieces -- rows 2h, 2h+1 of one column of a tile -- to the intermediate:
-    // consecutive threads take consecutive pieces of a tile row, so a tile is filled by 4
-    // lanes (2 with one column per block, the other column's block filling the rest)
-    const int cp = NCOL == 1 ? 0 : c;  // plane of this thread's column
+    // the block's rows into LDS in the intermediate's own tile order -- 8 float2 per tile
+    // row with two columns per block (column 0's 4 rows, then column 1's), the two halves
+    // swapped where bit 3 XOR bit 4 of the row is set, so that a wave's 8-byte writes (64
+    // consecutive rows of one column) fall in distinct banks; the rows in natural order with
+    // one column -- then 16-byte pieces (rows 2h, 2h+1 of one column of a tile) to the
+    // intermediate: thread u takes piece u of the block, and a tile row's pieces are one
+    // 64-byte run of LDS (a permutation of its 16-byte chunks), so the reads are conflict
+    // free; consecutive threads store consecutive pieces, a tile filled by 4 lanes (2 with
+    // one column per block, the other column's block filling the rest)
+    // (T >= 64 rows per step of i and T / 2 >= 32 tile rows per step of j leave bits 3
+    // and 4 of the row's parity unchanged: one swizzle per thread, immediate offsets)
+    static_assert(T >= 64, "swizzle constant over the store loops");
+    const int cp = NCOL == 1 ? 0 : c;
+    const int wslot = NCOL == 1 ? t : 8 * (t >> 2) + 4 * (cp ^ (((t >> 3) ^ (t >> 4)) & 1)) + (t & 3);
 #pragma unroll
-    for (int i = 0; i < V; ++i) xs[cp * HH + t + i * T] = v[0][i];
+    for (int i = 0; i < V; ++i) xs[wslot + (NCOL == 1 ? 1 : 2) * i * T] = v[0][i];
     lds_barrier();
     const int q = cp * T + t;  // thread index in the block (NCOL T threads)
     constexpr int PPT = TR / 2 * NCOL;  // pieces per tile written by this block
     float4* gout = reinterpret_cast<float4*>(Gb + (long)k * HH * Gs * CW) + (long)g * (TR * CW / 2);
+    const float4* xs4 = reinterpret_cast<const float4*>(xs);
+    const int rslot = NCOL == 1 ? q : q ^ (((((q / PPT) >> 1) ^ ((q / PPT) >> 2)) & 1) << 1);
 #pragma unroll
     for (int j = 0; j < HH / (2 * T); ++j) {
       const int u = q + j * NCOL * T;  // piece index: tile row u / PPT, piece u % PPT
@@ -558,9 +569,8 @@ __device__ __forceinline__ void colpass_four(float2 (&in)[NT][NZJ < V ? NZJ : V]
       const int col = NCOL == 1 ? c : (u % PPT) / (TR / 2);
       const int h2 = u % (TR / 2);
       const int y = trow * TR + 2 * h2;
-      const float2* src = xs + (NCOL == 1 ? 0 : col) * HH + y;
-      const float2 a = src[0], b = src[1];
-      if (FULL || y < ystore) gout[(long)trow * Gs * (TR * CW / 2) + tile_f4(col, h2)] = make_float4(a.x, a.y, b.x, b.y);
+      const float4 a = xs4[rslot + j * NCOL * T];
+      if (FULL || y < ystore) gout[(long)trow * Gs * (TR * CW / 2) + tile_f4(col, h2)] = a;
     }
   } else {
     // row y = t + i*T lies in tile row y / TR = t / TR + i * (T / TR) at position t % TR;
@@ -602,7 +612,7 @@ __global__ __launch_bounds__(CWB*(HH / V), CWB == 1 ? 4 : 1) void k_colpass(cons
   // transforms' work areas
   // TS: the transposed-store planes (one of HH float2 per column) after both
   constexpr int XS0 = CWB * NT * 2 * LF + (TWM == 1 ? FsTwLayout<HF>::floats : 0);
-  __shared__ float lds[XS0 + (TS ? 2 * CWB * HH : 0)];
+  __shared__ __attribute__((aligned(16))) float lds[XS0 + (TS ? 2 * CWB * HH : 0)];
   // block -> (tile column g, column c): the blocks that write the same 128-byte lines of
   // Gb -- tile columns 2m and 2m+1 (and both columns of each with one column per block)
   // -- get ids 8 apart, so they run on the same XCD (workgroups go to the 8 XCDs round
@@ -1687,7 +1697,7 @@ static void launch_colpass_n(tdtm_match_plan* p, int K, int ystore, hipStream_t 
   } while (0)
 #define TDTM_COLPASS(CWB_, NT_)                                                           \
   do {                                                                                    \
-    constexpr int xs_ = NT_ == 1 ? 2 * CWB_ * HH : 0; /* transposed-store planes */          \
+    constexpr int xs_ = NT_ == 1 ? 2 * CWB_ * HH : 0; /* transposed-store area */            \
     constexpr bool fits_ = FOUR && (CWB_ * NT_ * 2 * LFc + TWF + xs_) * 4 <= TDTM_LDS_BYTES;   \
     constexpr bool regs_ = FOUR && NT_ == 1;                                              \
     if (rtw && regs_) {                                                                   \

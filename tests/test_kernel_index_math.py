@@ -81,15 +81,40 @@ def test_batched_sync_staging_rounds(WS):
         assert sorted(dst) == [(r, x) for r in range(RB) for x in range(CW * G)]
 
 
+def _xs_slot(y, col, NCOL):
+    """colpass_four's LDS slot (float2) of row y of the block's column col."""
+    if NCOL == 1:
+        return y
+    return 8 * (y >> 2) + 4 * (col ^ (((y >> 3) ^ (y >> 4)) & 1)) + (y & 3)
+
+
+def _xs_piece(u, NCOL):
+    """The float4 slot thread u reads for piece u."""
+    if NCOL == 1:
+        return u
+    trow = u // (TR // 2 * NCOL)
+    return u ^ ((((trow >> 1) ^ (trow >> 2)) & 1) << 1)
+
+
 @pytest.mark.parametrize("HH", [512, 1024, 2048, 4096])
 @pytest.mark.parametrize("NCOL", [1, 2])
 def test_column_pass_pieces_cover_tiles_once(HH, NCOL):
     """colpass_four's transposed stores: thread q (NCOL * T threads) writes pieces
     u = q + j NCOL T, j < HH / (2 T): tile row u / PPT, column (NCOL == 1: the block's
     column c), rows 2 h2, 2 h2 + 1 -- together every (row, column) of the block's columns
-    once, 16-byte aligned, and 4 (2 with one column) consecutive threads per tile."""
+    once, 16-byte aligned, and 4 (2 with one column) consecutive threads per tile.  The LDS
+    slots the rows are written to are distinct and within NCOL * HH float2, and the 16-byte
+    slot a thread reads for its piece holds exactly that piece's two rows."""
     T = HH // V
     PPT = TR // 2 * NCOL
+    cols_all = [0] if NCOL == 1 else list(range(CW))
+    slot_of = {(y, cl): _xs_slot(y, cl, NCOL) for y in range(HH) for cl in cols_all}
+    assert sorted(slot_of.values()) == list(range(NCOL * HH))
+    for cl in cols_all:  # the writes' form: one slot per thread, immediate offsets
+        for t in range(T):
+            w = t if NCOL == 1 else 8 * (t >> 2) + 4 * (cl ^ (((t >> 3) ^ (t >> 4)) & 1)) + (t & 3)
+            for i in range(V):
+                assert w + (1 if NCOL == 1 else 2) * i * T == slot_of[(t + i * T, cl)]
     for c_block in range(CW if NCOL == 1 else 1):
         seen = []
         for q in range(NCOL * T):
@@ -104,8 +129,28 @@ def test_column_pass_pieces_cover_tiles_once(HH, NCOL):
                 seen += [(y, col), (y + 1, col)]
                 assert _tile_off(y % TR, col) // 2 == (col * TR + 2 * h2) // 2
                 assert f4 >= 0
+                lc = 0 if NCOL == 1 else col
+                s4 = _xs_piece(u, NCOL)
+                assert (slot_of[(y, lc)], slot_of[(y + 1, lc)]) == (2 * s4, 2 * s4 + 1)
+                # the kernel's form: one slot per thread plus the loop's immediate offset
+                rslot = q if NCOL == 1 else q ^ (((((q // PPT) >> 1) ^ ((q // PPT) >> 2)) & 1) << 1)
+                assert rslot + j * NCOL * T == s4
         cols = [c_block] if NCOL == 1 else list(range(CW))
         assert sorted(seen) == sorted((y, c) for y in range(HH) for c in cols)
+
+
+@pytest.mark.parametrize("HH", [512, 4096])
+def test_column_pass_lds_writes_spread_over_banks(HH):
+    """With two columns per block, a wave's 8-byte writes (64 consecutive rows of one
+    column) take 32 distinct float2 banks per 32 lanes (64 four-byte banks) and 16 per 16
+    lanes (32 banks)."""
+    for col in range(CW):
+        for y0 in range(0, HH, 64):
+            sl = [_xs_slot(y0 + l, col, 2) for l in range(64)]
+            for h in range(2):
+                assert len({x % 32 for x in sl[32 * h:32 * h + 32]}) == 32
+            for h in range(4):
+                assert len({x % 16 for x in sl[16 * h:16 * h + 16]}) == 16
 
 
 def test_mfma_pieces_cover_tiles_once():
@@ -120,3 +165,4 @@ def test_mfma_pieces_cover_tiles_once():
             y = trow * TR + 2 * h2
             seen += [(y, col), (y + 1, col)]
     assert sorted(seen) == sorted((y, c) for y in range(HH) for c in range(CW))
+
