@@ -2008,6 +2008,11 @@ int tdtm_match_plan_set_image_size(tdtm_match_plan_t p, int image_h, int image_w
   if (image_h < p->h || image_w < p->h || image_h > p->H || image_w > p->W)
     return fail("image size must lie between the template size and the plan's grid " + std::to_string(p->H) + "x" +
                 std::to_string(p->W));
+  if (image_h != p->Hn || image_w != p->Wn) {
+    // the captured steps carry the image size (projection normalisation, refinement crop)
+    release_graph(p);
+    release_refine_graphs(p);
+  }
   p->Hn = image_h;
   p->Wn = image_w;
   return 0;

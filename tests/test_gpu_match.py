@@ -334,6 +334,44 @@ def test_async_staging_refused_for_sub_wave_row_blocks(cuda_device):
     plan.set_option("rowpass_staging", 0)
 
 
+def test_image_size_change_recaptures_graph(cuda_device):
+    """A captured step carries the image size of the projection normalisation: after
+    tdtm_match_plan_set_image_size the next run must re-capture, matching a plan that had
+    that size from the start (launched directly), and differing from the old size."""
+    from leopard_em_amd import _lib
+    from leopard_em_amd.backend import MatchPlan, raw_statistics
+
+    print("image size change after a graph capture: 1024x1024 grid, 1000x900 image", flush=True)
+    args = _inputs(1024, 1024, 64, 2, 1, 8, seed=43)
+    img_dft, tdft, ctf, wf, dfs, pix, eul = args
+    tdft_d, eul_d = tdft.to(cuda_device), eul.to(cuda_device)
+    lib = _lib.load()
+
+    g = MatchPlan(1024, 1024, 64, 4, 2, 1, cuda_device)
+    g.set_inputs(img_dft, ctf, wf)
+    g.reset_stats()
+    g.run(tdft_d, eul_d, 0, 8)  # captured at 1024 x 1024
+    torch.cuda.synchronize()
+    before = raw_statistics(g)
+    _lib.check(lib.tdtm_match_plan_set_image_size(g.handle, 1000, 900))
+    g.reset_stats()
+    g.run(tdft_d, eul_d, 0, 8)
+    torch.cuda.synchronize()
+    after = raw_statistics(g)
+
+    d = MatchPlan(1024, 1024, 64, 4, 2, 1, cuda_device)
+    d.set_option("graph", 0)
+    _lib.check(lib.tdtm_match_plan_set_image_size(d.handle, 1000, 900))
+    d.set_inputs(img_dft, ctf, wf)
+    d.reset_stats()
+    d.run(tdft_d, eul_d, 0, 8)
+    torch.cuda.synchronize()
+    ref = raw_statistics(d)
+    for k in ref:
+        assert torch.equal(ref[k], after[k]), k
+    assert not torch.equal(before["mip"], after["mip"])
+
+
 def test_graph_replay_bit_identical_to_direct_steps(cuda_device):
     """tdtm_match_run with graph replays (default) against the same batches launched as
     plain steps: every statistics map bit-identical.  The graph is replayed across two
