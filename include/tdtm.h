@@ -59,7 +59,9 @@ int64_t tdtm_match_plan_bytes(tdtm_match_plan_t plan);
  * uses (normalize_template_projection's relative size and padded variance,
  * src/leopard_em/backend/utils.py:106-188); correlations at positions whose template
  * footprint lies inside the image are then those of the unpadded search.  Default: the
- * grid size.  No counterpart in the reference (torch.fft handles any size there).
+ * grid size.  A change releases the plan's captured step and refinement graphs (they are
+ * re-captured on the next run).  No counterpart in the reference (torch.fft handles any
+ * size there).
  */
 int tdtm_match_plan_set_image_size(tdtm_match_plan_t plan, int image_h, int image_w);
 
