@@ -246,7 +246,8 @@ def test_launch_options_do_not_change_results(opts, cuda_device):
         assert torch.equal(out[0][k], out[1][k]), k
 
 
-@pytest.mark.parametrize("opts", [{}, {"colpass_twiddles": 1}, {"colpass_store": 0}, {"colpass_persistent": 0}],
+@pytest.mark.parametrize("opts", [{}, {"colpass_twiddles": 1}, {"colpass_store": 0}, {"colpass_persistent": 0},
+                                  {"colpass_columns": 1}, {"colpass_columns": 1, "colpass_persistent": 0}],
                          ids=lambda o: ",".join(f"{k}={v}" for k, v in o.items()) or "defaults")
 def test_launch_options_do_not_change_results_4096(opts, cuda_device):
     """The production image size (eight-wave four-step column transforms)."""
