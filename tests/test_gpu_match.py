@@ -373,6 +373,49 @@ def test_image_size_change_recaptures_graph(cuda_device):
     assert not torch.equal(before["mip"], after["mip"])
 
 
+def test_magnification_and_option_changes_recapture_graph(cuda_device):
+    """Captured steps carry the slice kernel's magnification and the launch shapes: after
+    set_magnification or set_option the next run re-captures and matches a plan that had
+    the setting from the start (launched directly)."""
+    from leopard_em_amd.backend import MatchPlan, raw_statistics
+
+    print("magnification / option change after a graph capture: 512x512 h64 D2 O8", flush=True)
+    img_dft, tdft, ctf, wf, dfs, pix, eul = _inputs(512, 512, 64, 2, 1, 8, seed=47)
+    tdft_d, eul_d = tdft.to(cuda_device), eul.to(cuda_device)
+    mag = [[1.02, 0.01], [-0.015, 0.99]]
+
+    def run(plan):
+        plan.reset_stats()
+        plan.run(tdft_d, eul_d, 0, 8)
+        torch.cuda.synchronize()
+        return raw_statistics(plan)
+
+    def fresh(graph, **settings):
+        p = MatchPlan(512, 512, 64, 4, 2, 1, cuda_device)
+        p.set_option("graph", graph)
+        if "mag" in settings:
+            p.set_magnification(settings["mag"])
+        for k, v in settings.get("opts", {}).items():
+            p.set_option(k, v)
+        p.set_inputs(img_dft, ctf, wf)
+        return p
+
+    g = fresh(1)
+    plain = run(g)                      # captured without magnification
+    g.set_magnification(mag)
+    with_mag = run(g)
+    ref = run(fresh(0, mag=mag))
+    for k in ref:
+        assert torch.equal(ref[k], with_mag[k]), k
+    assert not torch.equal(plain["mip"], with_mag["mip"])
+    g.set_option("rowpass_rows", 2)     # another launch shape, same results
+    assert all(torch.equal(with_mag[k], v) for k, v in run(g).items())
+    g.set_magnification(None)
+    back = run(g)
+    for k in plain:
+        assert torch.equal(plain[k], back[k]), k
+
+
 def test_graph_replay_bit_identical_to_direct_steps(cuda_device):
     """tdtm_match_run with graph replays (default) against the same batches launched as
     plain steps: every statistics map bit-identical.  The graph is replayed across two
